@@ -1,0 +1,184 @@
+"""Benchmark of the OpenSayal per-step solver on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload full|small]
+
+A step is one Fluid::update of the wind-tunnel workload: 1920x1080 cells,
+projection n = 50, o = 1.9, FP32, seeded +-0.01 velocity noise (synthetic
+input, built on the host and uploaded before timing).  W untimed steps, then
+exactly K steps between device synchronisations; prints one JSON line.
+
+roofline: the dominant kernel is the projection.  Its algorithmic traffic is
+17 B per cell per sweep (u and v read and written once, 4 B each, plus the
+1-byte cell mask), so one launch that runs s sweeps over the grid moves
+17 * s * W * H bytes; `achieved` divides that by the launch's average
+duration, timed with HIP events on the stream the launches run on.
+cpu_baseline: the oracle (a C restatement of the reference's update rule,
+OpenMP over rows) timed on this host's cores on a bounded number of steps.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "Gcell-steps/s at 1920x1080 n=50; % HBM roofline; p50/p99 batched step latency"
+
+
+def step_bytes_per_cell(n: int) -> int:
+    """Algorithmic bytes per cell-step, FP32, pressure off: forces 8 (v read
+    and written), 17 per sweep, u/v advection 17, smoke advection 17."""
+    return 42 + 17 * n
+
+
+def host_threads() -> int:
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(n, 16))
+
+
+def cpu_baseline(cfg, seed: int, budget_s: float = 12.0) -> dict:
+    """Times the oracle (FP32, OpenMP) on this host for a bounded sample."""
+    from opensayal_amd import workloads as wl
+    from oracle.oracle import OracleFluid
+
+    threads = host_threads()
+    o = OracleFluid(cfg, nthreads=threads)
+    u, v, s = wl.initial_state(cfg, seed)
+    o.set_state(u=u, v=v, smoke=s)
+    t0 = time.perf_counter()
+    o.update(steps=1)
+    first = time.perf_counter() - t0
+    steps = max(2, min(40, int(budget_s / max(first, 1e-6))))
+    t0 = time.perf_counter()
+    o.update(steps=steps)
+    dt = time.perf_counter() - t0
+    cells = cfg.width * cfg.height
+    return {
+        "value": cells * steps / dt / 1e9,
+        "unit": "Gcell-steps/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{steps} steps of the same {cfg.width}x{cfg.height} n={cfg.projection_n} "
+                  f"workload after 1 untimed step ({dt:.1f} s), oracle FP32, "
+                  f"OpenMP {threads} threads",
+        "ms_per_step": dt / steps * 1e3,
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=300)
+    ap.add_argument("--warmup", type=int, default=30)
+    ap.add_argument("--workload", default="full", choices=["full", "small"])
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--kernel-reps", type=int, default=20)
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build measures one GPU only; each simulation runs on a single "
+              "device (multi-GPU sharding is described in DESIGN.md, not built)", file=sys.stderr)
+        return 2
+
+    import torch
+
+    from opensayal_amd import Fluid, workloads as wl
+
+    cfg = wl.plain_full() if args.workload == "full" else wl.plain_small()
+    W, H, n = cfg.width, cfg.height, cfg.projection_n
+    fl = Fluid(cfg)
+    u, v, s = wl.initial_state(cfg, args.seed)
+    fl.set_field("vel_x", u)
+    fl.set_field("vel_y", v)
+    fl.set_field("smoke", s)
+
+    stream = torch.cuda.Stream()
+    handle = stream.cuda_stream
+    fl.step_async(args.warmup, stream=handle)
+    torch.cuda.synchronize()
+
+    # timed region: exactly K steps
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    start.record(stream)
+    fl.step_async(args.steps, stream=handle)
+    end.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    gpu_ms = start.elapsed_time(end)
+    ms_per_step = gpu_ms / args.steps
+    value = W * H * args.steps / (gpu_ms / 1e3) / 1e9
+
+    # dominant kernel: projection launches timed one call at a time with
+    # events on the stream they run on
+    spl = fl.sweeps_per_launch
+    durations = []
+    for _ in range(args.kernel_reps):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        fl.project(sweeps=spl, stream=handle, sync=False)
+        b.record(stream)
+        durations.append((a, b))
+    torch.cuda.synchronize()
+    per_call_ms = sum(a.elapsed_time(b) for a, b in durations) / len(durations)
+    launches = fl.launches_per_call(spl)
+    per_launch_ms = per_call_ms / launches
+    alg_bytes_per_launch = 17 * spl * W * H / launches
+    achieved = alg_bytes_per_launch / (per_launch_ms / 1e3) / 1e9
+    roofline = {
+        "bound": "hbm",
+        "achieved": round(achieved, 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": None,
+        "kernel": fl.projection_kernel_name,
+        "avg_launch_us": round(per_launch_ms * 1e3, 2),
+        "alg_bytes_per_launch": int(alg_bytes_per_launch),
+    }
+    step_alg = step_bytes_per_cell(n) * W * H * args.steps / (gpu_ms / 1e3) / 1e9
+    out = {
+        "metric": METRIC,
+        "value": round(value, 4),
+        "unit": "Gcell-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: seeded +-0.01 velocity noise, smoke 1 on the inflow",
+        "config": {"workload": f"plain_{args.workload}_{W}x{H}_n{n}", "width": W, "height": H,
+                   "projection_n": n, "projection_o": round(cfg.projection_o, 4),
+                   "pipe_height": cfg.pipe_height, "obstacle_radius": cfg.obstacle_radius,
+                   "seed": args.seed},
+        "roofline": roofline,
+        "step_roofline": {"alg_bytes_per_cell_step": step_bytes_per_cell(n),
+                          "achieved_GBs": round(step_alg, 1),
+                          "frac": round(step_alg / HBM_PEAK_GBS, 4)},
+        "wall_s": round(wall, 3),
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(cfg, args.seed)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+    fl.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,201 @@
+/*
+ * opensayal.h - C ABI of the MI355X (gfx950) OpenSayal per-step fluid solver.
+ *
+ * The reference exposes the solver as the C++ class `Fluid`
+ * (inc/fluid.cuh:15-116): construction from a parsed `Config`
+ * (src/fluid.cu:41-71, src/config_parser.cpp:15-184), one step per call to
+ * `Fluid::update(Source, float d_t)` (src/fluid.cu:770-795), and public
+ * device pointers plus `min_pressure`/`max_pressure` host members
+ * (inc/fluid.cuh:61-72).  This header replaces that class with an opaque
+ * handle and plain C entry points: no C++ types, no exceptions, every call
+ * returns an `osy_status` (0 = ok, negative = error) and `osy_last_error()`
+ * gives the message of the calling thread's last failure.
+ *
+ * Field layout at this boundary is the reference's: a W x H array indexed by
+ * indx(i, j) = (H - 1 - j) * W + i (src/fluid.cu:163-165), memory row 0 being
+ * the top row.  u(i, j) sits on the left face of cell (i, j), v(i, j) on its
+ * bottom face (src/fluid.cu:552-558).
+ *
+ * Threading: a handle is not thread-safe; distinct handles may be used from
+ * distinct threads.  All handles of one process share the current HIP device.
+ */
+#ifndef OPENSAYAL_H
+#define OPENSAYAL_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OSY_ABI_VERSION 1
+
+typedef enum osy_status {
+  OSY_OK = 0,
+  OSY_E_INVALID = -1,     /* bad argument or configuration value            */
+  OSY_E_UNSUPPORTED = -2, /* a configuration the solver does not implement  */
+  OSY_E_HIP = -3,         /* a HIP runtime call failed                      */
+  OSY_E_NOMEM = -4,       /* device or host allocation failed               */
+  OSY_E_CONFIG = -5,      /* configuration file unreadable or malformed     */
+  OSY_E_BUDGET = -6,      /* request does not fit the device-memory budget  */
+  OSY_E_STATE = -7        /* call not valid in the object's current state   */
+} osy_status;
+
+/* Storage type of u, v and smoke.  Pressure and all arithmetic stay FP32. */
+typedef enum osy_storage {
+  OSY_STORAGE_F32 = 0,
+  OSY_STORAGE_F16 = 1,
+  OSY_STORAGE_BF16 = 2
+} osy_storage;
+
+/* Field identifiers for get/set.  Element type in parentheses. */
+typedef enum osy_field {
+  OSY_FIELD_VEL_X = 0,    /* u, d_vel_x (float)                            */
+  OSY_FIELD_VEL_Y = 1,    /* v, d_vel_y (float)                            */
+  OSY_FIELD_SMOKE = 2,    /* d_smoke (float)                               */
+  OSY_FIELD_PRESSURE = 3, /* d_pressure (float)                            */
+  OSY_FIELD_IS_SOLID = 4, /* d_is_solid (int32, 0/1); read-only            */
+  OSY_FIELD_TOTAL_S = 5   /* d_total_s (int32, 0..4); read-only            */
+} osy_field;
+
+/*
+ * Solver configuration: the solver-relevant part of the reference's `Config`
+ * (inc/config_parser.hpp:9-116).  Names follow the OpenSayal.conf.json keys.
+ * Keys that only affect rendering, the CUDA launch shape or OpenMP are
+ * accepted by the JSON reader and kept here for completeness; they do not
+ * change the solver's results.
+ */
+typedef struct osy_config {
+  /* sim.* */
+  int32_t width;              /* sim.width, default 1920                      */
+  int32_t height;             /* sim.height, default 1080                     */
+  float cell_size;            /* sim.cell_size, default 1 (truncated to int)  */
+  int32_t cell_pixel_size;    /* sim.cell_pixel_size (render only)            */
+  int32_t enable_drain;       /* sim.enable_drain, default 1                  */
+  int32_t enable_pressure;    /* sim.enable_pressure, default 0               */
+  int32_t enable_smoke;       /* sim.enable_smoke, default 1                  */
+  int32_t enable_interactive; /* sim.enable_interactive (mouse; no effect)    */
+  int32_t projection_n;       /* sim.projection.n, default 50                 */
+  float projection_o;         /* sim.projection.o, default 1.9                */
+  int32_t pipe_height;        /* sim.wind_tunnel.pipe_height, default H/4     */
+  int32_t pipe_length;        /* not configurable in the reference: always 0  */
+  int32_t smoke_length;       /* sim.wind_tunnel.smoke_length, default 1      */
+  int32_t smoke_height;       /* sim.wind_tunnel.smoke_height, default H/4    */
+  int32_t smoke_count;        /* sim.wind_tunnel.smoke_count, default 1       */
+  float wind_speed;           /* sim.wind_tunnel.speed, default 0             */
+  float wind_smoke;           /* sim.wind_tunnel.smoke, default 1             */
+  float g;                    /* sim.physics.g, default 0                     */
+  float d_t;                  /* sim.time.d_t, default 0.05                   */
+  int32_t enable_real_time;   /* sim.time.enable_real_time (headless: no-op)  */
+  float real_time_multiplier; /* sim.time.real_time_multiplier                */
+  int32_t enable_decay;       /* sim.smoke.enable_decay, default 0            */
+  float decay_rate;           /* sim.smoke.decay_rate, default 0.05           */
+  int32_t obstacle_enable;    /* sim.obstacle.enable, default 1               */
+  int32_t obstacle_center_x;  /* sim.obstacle.center_x, default W/2           */
+  int32_t obstacle_center_y;  /* sim.obstacle.center_y, default H/2           */
+  float obstacle_radius;      /* sim.obstacle.radius, default min(W,H)/30     */
+  /* fluid.* */
+  float density;              /* fluid.density, default 1                     */
+  float drag_coeff;           /* fluid.drag_coeff, default 0                  */
+  float viscosity;            /* fluid.viscosity, default 0.001; must be 0    */
+  /* thread.* (accepted, no effect) */
+  int32_t block_size_x;       /* thread.cuda.block_size_x, default 64         */
+  int32_t block_size_y;       /* thread.cuda.block_size_y, default 1          */
+  int32_t omp_thread_count;   /* thread.openMP.thread_count                   */
+  /* this implementation's own key "sim.storage": "f32" | "f16" | "bf16"    */
+  int32_t storage;            /* osy_storage, default OSY_STORAGE_F32         */
+} osy_config;
+
+/* Mouse source of the reference (inc/fluid.cuh:8-13).  Headless callers pass
+ * NULL or active = 0. */
+typedef struct osy_source {
+  int32_t active;
+  float smoke;
+  float velocity;
+  int32_t position_x;
+  int32_t position_y;
+} osy_source;
+
+/* Geometry and device storage of a created solver. */
+typedef struct osy_fluid_info {
+  int32_t width, height, cell_size, projection_n;
+  int32_t pitch;        /* elements per device row                        */
+  int32_t storage;      /* osy_storage of u, v, smoke                      */
+  int32_t projection_kernel; /* 0: one launch per colour phase;
+                                1: LDS-tiled, several sweeps per launch    */
+  int32_t sweeps_per_launch; /* sweeps one projection launch runs          */
+  size_t device_bytes;  /* device memory held by this handle               */
+} osy_fluid_info;
+
+typedef struct osy_fluid osy_fluid;
+
+/* ---- configuration: replaces ConfigParser (src/config_parser.cpp:11-184) */
+
+/* Fill `cfg` with the reference's defaults (src/config_parser.cpp:21-181),
+ * with viscosity forced to 0 (the reference's diffusion is not supported). */
+int osy_config_defaults(osy_config* cfg);
+/* ConfigParser(path).parse(): dotted or nested keys, get_or semantics
+ * (inc/config_parser.hpp:131-150). */
+int osy_config_parse_file(const char* path, osy_config* cfg);
+/* Same, from JSON text in memory. */
+int osy_config_parse_string(const char* json_text, osy_config* cfg);
+
+/* ---- the solver: replaces class Fluid (inc/fluid.cuh:15-116) ---------- */
+
+/* Fluid::Fluid(Config) (src/fluid.cu:41-161): allocates device state, builds
+ * the solid mask and neighbour counts, zeroes u, v, smoke and pressure. */
+int osy_fluid_create(const osy_config* cfg, osy_fluid** out);
+/* Fluid(ConfigParser(path).parse()). */
+int osy_fluid_create_from_json(const char* path, osy_fluid** out);
+/* Fluid::~Fluid (src/fluid.cu:73-84).  NULL is accepted. */
+int osy_fluid_destroy(osy_fluid* fluid);
+
+/* Fluid::update(Source, d_t) (src/fluid.cu:770-795): one step, synchronous.
+ * `src` may be NULL (inactive source). */
+int osy_fluid_update(osy_fluid* fluid, const osy_source* src, float d_t);
+/* `steps` calls of Fluid::update with an inactive source, replayed from a
+ * captured HIP graph; synchronous. */
+int osy_fluid_step(osy_fluid* fluid, int32_t steps, float d_t);
+/* As osy_fluid_step but enqueued on `hip_stream` (a hipStream_t, NULL = the
+ * handle's own stream) and returning without waiting. */
+int osy_fluid_step_async(osy_fluid* fluid, int32_t steps, float d_t,
+                         void* hip_stream);
+/* Wait for all work enqueued on the handle's stream. */
+int osy_fluid_synchronize(osy_fluid* fluid);
+
+/* Fluid::apply_projection (src/fluid.cu:282-295) alone: `sweeps` red-black
+ * sweeps on the current u, v (and pressure accumulation when enabled; the
+ * pressure is not zeroed first).  Enqueued on `hip_stream` (NULL = the
+ * handle's stream) without waiting.  For timing and testing the dominant
+ * phase by itself. */
+int osy_fluid_project(osy_fluid* fluid, int32_t sweeps, void* hip_stream);
+
+/* Copy a field to / from host memory in the reference's indx order.
+ * `bytes` must equal W*H*4.  IS_SOLID and TOTAL_S are read-only. */
+int osy_fluid_get_field(osy_fluid* fluid, int32_t field, void* dst,
+                        size_t bytes);
+int osy_fluid_set_field(osy_fluid* fluid, int32_t field, const void* src,
+                        size_t bytes);
+
+/* Fluid::min_pressure / max_pressure after the last step (thrust reduce at
+ * src/fluid.cu:778-787).  Both are 0 when pressure is disabled. */
+int osy_fluid_pressure_range(osy_fluid* fluid, float* min_pressure,
+                             float* max_pressure);
+/* L2 and max-abs norms, accumulated in FP64, of the divergence
+ * ((u(i+1,j) - u(i,j)) + v(i,j+1)) - v(i,j) over interior fluid cells, the
+ * formula of Fluid::get_divergence (src/fluid.cu:192-201). */
+int osy_fluid_divergence_norm(osy_fluid* fluid, double* l2, double* linf);
+
+int osy_fluid_get_info(const osy_fluid* fluid, osy_fluid_info* info);
+
+/* Message of the calling thread's last failed call ("" if none). */
+const char* osy_last_error(void);
+/* OSY_ABI_VERSION of the loaded library. */
+int osy_abi_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* OPENSAYAL_H */

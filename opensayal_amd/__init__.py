@@ -1,0 +1,184 @@
+"""OpenSayal per-step fluid solver for AMD Instinct MI355X (gfx950).
+
+Python mirror of the reference's solver interface over the C ABI of
+``libopensayal_amd.so`` (declared in include/opensayal.h):
+
+* ``ConfigParser(path).parse()`` -> ``Config`` (src/config_parser.cpp:11-184)
+* ``Fluid(config)`` (src/fluid.cu:41-71), ``Fluid.update(source, d_t)``
+  (src/fluid.cu:770-795), ``min_pressure`` / ``max_pressure`` (inc/fluid.cuh:61-62)
+* ``Source`` (inc/fluid.cuh:8-13)
+
+Fields come back as numpy arrays of shape (H, W) in the reference's memory
+order: ``a[r, i]`` is cell ``(i, j = H - 1 - r)`` (Fluid::indx,
+src/fluid.cu:163-165).  Every compute call runs the HIP kernels; there is no
+CPU fallback, and a missing library raises ``OpenSayalError``.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional
+
+import numpy as np
+
+from ._abi import (FIELDS, STORAGE, OpenSayalError, OsyConfig, OsyFluidInfo, OsySource,
+                   check, lib)
+
+__all__ = ["Config", "ConfigParser", "Fluid", "Source", "OpenSayalError", "default_config",
+           "FIELDS", "STORAGE"]
+
+Config = OsyConfig
+Source = OsySource
+
+
+def default_config(width: int = 1920, height: int = 1080, **overrides) -> Config:
+    """The reference's defaults (src/config_parser.cpp:21-181) for a W x H grid.
+
+    Defaults that depend on the grid size (pipe and smoke heights, obstacle
+    centre and radius) follow ``width``/``height``.  ``viscosity`` is 0.
+    ``overrides`` are ``Config`` field names.
+    """
+    text = '{"sim": {"width": %d, "height": %d}, "fluid": {"viscosity": 0}}' % (width, height)
+    c = Config()
+    check(lib().osy_config_parse_string(text.encode(), ctypes.byref(c)))
+    for k, v in overrides.items():
+        if k == "storage" and isinstance(v, str):
+            v = STORAGE[v]
+        if not hasattr(c, k):
+            raise AttributeError(f"Config has no field {k!r}")
+        setattr(c, k, v)
+    return c
+
+
+class ConfigParser:
+    """ConfigParser (src/config_parser.cpp:11-13): reads OpenSayal.conf.json."""
+
+    def __init__(self, config_file_name: str = "OpenSayal.conf.json"):
+        self.config_file_name = config_file_name
+
+    def parse(self) -> Config:
+        c = Config()
+        check(lib().osy_config_parse_file(os.fsencode(self.config_file_name), ctypes.byref(c)))
+        return c
+
+    @staticmethod
+    def parse_string(text: str) -> Config:
+        c = Config()
+        check(lib().osy_config_parse_string(text.encode(), ctypes.byref(c)))
+        return c
+
+
+class Fluid:
+    """The solver state of one grid on the current HIP device (class Fluid,
+    inc/fluid.cuh:15-116).  Owns its device memory until ``close()``."""
+
+    def __init__(self, config: Config):
+        self._h = ctypes.c_void_p()
+        check(lib().osy_fluid_create(ctypes.byref(config), ctypes.byref(self._h)))
+        self.config = config.copy()
+        self._refresh_info()
+
+    def _refresh_info(self) -> None:
+        info = OsyFluidInfo()
+        check(lib().osy_fluid_get_info(self._h, ctypes.byref(info)))
+        self.width, self.height = info.width, info.height
+        self.pitch = info.pitch
+        self.device_bytes = info.device_bytes
+        self.projection_kernel = info.projection_kernel
+        self.sweeps_per_launch = info.sweeps_per_launch
+
+    @property
+    def projection_kernel_name(self) -> str:
+        return {0: "projection_colour", 1: "projection_tiled"}.get(self.projection_kernel, "?")
+
+    def launches_per_call(self, sweeps: int) -> int:
+        """Projection kernel launches one ``project(sweeps)`` call makes."""
+        if self.projection_kernel == 0:
+            return 2 * sweeps
+        k = self.sweeps_per_launch
+        return -(-sweeps // k)
+
+    @classmethod
+    def from_json(cls, path: str) -> "Fluid":
+        return cls(ConfigParser(path).parse())
+
+    # -- lifetime ----------------------------------------------------------
+    def close(self) -> None:
+        if self._h:
+            check(lib().osy_fluid_destroy(self._h))
+            self._h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- stepping ----------------------------------------------------------
+    def update(self, source: Optional[Source] = None, d_t: Optional[float] = None) -> None:
+        """Fluid::update(Source, d_t): one synchronous step."""
+        dt = self.config.d_t if d_t is None else d_t
+        src = ctypes.byref(source) if source is not None else None
+        check(lib().osy_fluid_update(self._h, src, ctypes.c_float(dt)))
+
+    def step(self, steps: int, d_t: Optional[float] = None) -> None:
+        """``steps`` updates with an inactive source, replayed from a HIP graph."""
+        dt = self.config.d_t if d_t is None else d_t
+        check(lib().osy_fluid_step(self._h, int(steps), ctypes.c_float(dt)))
+
+    def step_async(self, steps: int, d_t: Optional[float] = None, stream: int = 0) -> None:
+        dt = self.config.d_t if d_t is None else d_t
+        check(lib().osy_fluid_step_async(self._h, int(steps), ctypes.c_float(dt),
+                                         ctypes.c_void_p(stream or None)))
+
+    def synchronize(self) -> None:
+        check(lib().osy_fluid_synchronize(self._h))
+
+    def project(self, sweeps: Optional[int] = None, stream: int = 0, sync: bool = True) -> None:
+        """Fluid::apply_projection alone (``sweeps`` defaults to projection n)."""
+        n = self.config.projection_n if sweeps is None else sweeps
+        check(lib().osy_fluid_project(self._h, int(n), ctypes.c_void_p(stream or None)))
+        if sync:
+            self.synchronize()
+
+    # -- state access ------------------------------------------------------
+    def get_field(self, name: str) -> np.ndarray:
+        fid = FIELDS[name]
+        dtype = np.int32 if name in ("is_solid", "total_s") else np.float32
+        out = np.empty((self.height, self.width), dtype=dtype)
+        check(lib().osy_fluid_get_field(self._h, fid, out.ctypes.data_as(ctypes.c_void_p),
+                                        out.nbytes))
+        return out
+
+    def set_field(self, name: str, values: np.ndarray) -> None:
+        a = np.ascontiguousarray(values, dtype=np.float32)
+        if a.shape != (self.height, self.width):
+            raise ValueError(f"expected shape {(self.height, self.width)}, got {a.shape}")
+        check(lib().osy_fluid_set_field(self._h, FIELDS[name], a.ctypes.data_as(ctypes.c_void_p),
+                                        a.nbytes))
+
+    @property
+    def pressure_range(self) -> tuple:
+        lo, hi = ctypes.c_float(), ctypes.c_float()
+        check(lib().osy_fluid_pressure_range(self._h, ctypes.byref(lo), ctypes.byref(hi)))
+        return lo.value, hi.value
+
+    @property
+    def min_pressure(self) -> float:
+        return self.pressure_range[0]
+
+    @property
+    def max_pressure(self) -> float:
+        return self.pressure_range[1]
+
+    def divergence_norm(self) -> tuple:
+        """(L2, max-abs) of the divergence over interior fluid cells, in FP64."""
+        l2, mx = ctypes.c_double(), ctypes.c_double()
+        check(lib().osy_fluid_divergence_norm(self._h, ctypes.byref(l2), ctypes.byref(mx)))
+        return l2.value, mx.value

@@ -1,0 +1,84 @@
+"""Builds the solver library ``libopensayal_amd.so`` in-tree with hipcc for gfx950.
+
+Every translation unit is compiled with ``-ffp-contract=off`` (no fused
+multiply-add, so float arithmetic rounds after each operation as the
+reference's source states) and without fast-math (division and square root
+stay correctly rounded).  ``-packed-fp32-ops`` is switched off for device code.
+
+Usage: ``python -m opensayal_amd.build`` (also called by ``__graft_entry__.build``).
+"""
+from __future__ import annotations
+
+import concurrent.futures
+import os
+import shutil
+import subprocess
+import sys
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+CSRC = PKG / "csrc"
+LIB = PKG / "libopensayal_amd.so"
+OBJ = PKG / "_build"
+
+SOURCES = [
+    "json_value.cpp",
+    "config.cpp",
+    "fluid_host.cpp",
+    "kernels_step.hip",
+    "kernels_project.hip",
+]
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+
+COMMON = [
+    "-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",
+    "-ffp-contract=off", "-fno-fast-math",
+]
+DEVICE = [
+    f"--offload-arch={ARCH}",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-fno-gpu-rdc",
+]
+
+
+def _compile(src: str) -> Path:
+    out = OBJ / (Path(src).stem + ".o")
+    path = CSRC / src
+    if src.endswith(".hip"):
+        cmd = [HIPCC, *COMMON, *DEVICE, "-x", "hip", "-c", str(path), "-o", str(out)]
+    else:
+        cmd = [HIPCC, *COMMON, "-c", str(path), "-o", str(out)]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
+    return out
+
+
+def build(verbose: bool = True) -> Path:
+    """Compiles every source and links the shared library; returns its path."""
+    OBJ.mkdir(exist_ok=True)
+    with concurrent.futures.ThreadPoolExecutor(max_workers=min(8, len(SOURCES))) as ex:
+        objs = list(ex.map(_compile, SOURCES))
+    tmp = LIB.with_suffix(".so.tmp")
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp),
+           *[str(o) for o in objs]]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"link failed: {' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
+    os.replace(tmp, LIB)
+    if verbose:
+        print(f"built {LIB}", file=sys.stderr)
+    return LIB
+
+
+def clean() -> None:
+    if OBJ.exists():
+        shutil.rmtree(OBJ)
+    if LIB.exists():
+        LIB.unlink()
+
+
+if __name__ == "__main__":
+    build()

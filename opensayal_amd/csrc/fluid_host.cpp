@@ -1,0 +1,429 @@
+// fluid_host.cpp - the solver object behind the C ABI: replaces the
+// reference's class Fluid (inc/fluid.cuh:15-116, src/fluid.cu:41-161, 770-795)
+// with device state laid out for gfx950, one HIP stream per handle and the
+// step sequence replayed from captured HIP graphs.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "kernels.hpp"
+#include "osy_internal.hpp"
+#include "projection.hpp"
+
+namespace osy {
+
+namespace {
+thread_local std::string g_last_error;
+}
+
+int set_error(int status, const std::string& msg) {
+  g_last_error = msg;
+  return status;
+}
+void clear_error() { g_last_error.clear(); }
+
+#define OSY_HIP_CHECK(call)                                                              \
+  do {                                                                                   \
+    hipError_t e_ = (call);                                                              \
+    if (e_ != hipSuccess)                                                                \
+      return ::osy::set_error(OSY_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+int config_validate(const osy_config& c) {
+  if (c.width < 3 || c.height < 3 || c.width > 32768 || c.height > 32768)
+    return set_error(OSY_E_INVALID, "sim.width and sim.height must be in [3, 32768]");
+  const int cs = (int)c.cell_size;  // Fluid::cell_size is int (inc/fluid.cuh:46)
+  if (!(c.cell_size >= 1.0f) || cs > 64)
+    return set_error(OSY_E_INVALID, "sim.cell_size must be in [1, 64] (it is truncated to int)");
+  if (c.projection_n < 0 || c.projection_n > 100000)
+    return set_error(OSY_E_INVALID, "sim.projection.n must be in [0, 100000]");
+  if (!(c.projection_o >= 0.0f && c.projection_o <= 2.0f))
+    return set_error(OSY_E_INVALID, "sim.projection.o must be within [0, 2]");
+  if (c.viscosity != 0.0f)
+    return set_error(OSY_E_UNSUPPORTED,
+                     "fluid.viscosity must be 0: the reference's diffusion pass "
+                     "(src/fluid.cu:167-190) is racy and reads an uninitialised "
+                     "loop counter, so it has no defined result to reproduce");
+  if (c.smoke_count != 1) {
+    const int spacing = (c.smoke_count - 1) != 0
+                            ? (c.pipe_height - c.smoke_count * c.smoke_height) / (c.smoke_count - 1)
+                            : 0;
+    if (spacing + c.smoke_height == 0)
+      return set_error(OSY_E_INVALID,
+                       "sim.wind_tunnel: smoke stripe period is 0 (the reference divides by zero)");
+  }
+  if (std::abs((long long)c.obstacle_center_x) > (1LL << 24) ||
+      std::abs((long long)c.obstacle_center_y) > (1LL << 24) ||
+      std::fabs(c.obstacle_radius) >= 16777216.0f)
+    return set_error(OSY_E_INVALID, "sim.obstacle centre and radius must be below 2^24");
+  if (!std::isfinite(c.d_t) || !std::isfinite(c.g) || !std::isfinite(c.wind_speed) ||
+      !std::isfinite(c.density))
+    return set_error(OSY_E_INVALID, "d_t, g, wind_tunnel.speed and density must be finite");
+  if (c.storage < OSY_STORAGE_F32 || c.storage > OSY_STORAGE_BF16)
+    return set_error(OSY_E_INVALID, "storage must be f32, f16 or bf16");
+  return OSY_OK;
+}
+
+}  // namespace osy
+
+using namespace osy;
+
+struct osy_fluid {
+  osy_config cfg{};
+  int W = 0, H = 0, P = 0, cs = 1, storage = 0, elem = 4;
+  FieldPtrs f{};
+  int uv = 0, sm = 0;
+  hipStream_t stream = nullptr;
+  int device = 0;
+  void* block = nullptr;     // one device allocation holding every buffer
+  size_t block_bytes = 0;
+  float* staging = nullptr;  // W*H*4 bytes, reference-order copies
+  int projection_kernel = PROJ_AUTO;
+  struct Graph {
+    hipGraphExec_t exec = nullptr;
+    float d_t = 0.0f;
+    int uv_out = 0, sm_out = 0;
+  } graphs[2][2];
+
+  ~osy_fluid() {
+    for (auto& row : graphs)
+      for (auto& g : row)
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    if (block) (void)hipFree(block);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  // StepParams for this configuration and step size; `src` may be null.
+  StepParams params(float d_t, const osy_source* src) const {
+    const osy_config& c = cfg;
+    StepParams sp{};
+    sp.W = W; sp.H = H; sp.P = P; sp.cs = cs;
+    sp.n = c.projection_n;
+    sp.o = c.projection_o;
+    sp.density = c.density;
+    sp.d_t = d_t;
+    sp.smoke_length = c.smoke_length;
+    sp.band_lo = H / 2 - c.pipe_height / 2;
+    sp.band_hi = H / 2 + c.pipe_height / 2;
+    sp.smoke_count = c.smoke_count;
+    sp.smoke_height = c.smoke_height;
+    if (c.smoke_count != 1) {
+      const int spacing = (c.pipe_height - c.smoke_count * c.smoke_height) / (c.smoke_count - 1);
+      sp.smoke_mod = spacing + c.smoke_height;
+    } else {
+      sp.smoke_mod = 1;
+    }
+    sp.smoke_top = H / 2 + c.pipe_height / 2;
+    sp.sband_lo = H / 2 - c.smoke_height / 2;
+    sp.sband_hi = H / 2 + c.smoke_height / 2;
+    sp.speed = c.wind_speed;
+    sp.wind_smoke = c.wind_smoke;
+    sp.drag_on = c.drag_coeff != 0.0f;
+    // expf(-drag_coeff * d_t) (src/fluid.cu:332) is one value per step;
+    // it is evaluated once here with the host's expf.
+    sp.damping = sp.drag_on ? expf(-c.drag_coeff * d_t) : 1.0f;
+    sp.gdt = c.g * d_t;
+    sp.src_active = src && src->active;
+    if (sp.src_active) {
+      sp.src_x = src->position_x;
+      sp.src_y = src->position_y;
+      sp.src_smoke = src->smoke;
+      sp.src_velocity = src->velocity;
+    }
+    sp.pressure_on = c.enable_pressure != 0;
+    sp.smoke_on = c.enable_smoke && c.wind_smoke != 0.0f;
+    sp.decay_on = c.enable_decay != 0;
+    sp.decay_rd = c.decay_rate * d_t;
+    return sp;
+  }
+};
+
+namespace {
+
+size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+int create_impl(const osy_config* cfg, osy_fluid** out) {
+  if (!cfg || !out) return set_error(OSY_E_INVALID, "null argument");
+  *out = nullptr;
+  int st = config_validate(*cfg);
+  if (st != OSY_OK) return st;
+  int ndev = 0;
+  OSY_HIP_CHECK(hipGetDeviceCount(&ndev));
+  if (ndev <= 0) return set_error(OSY_E_HIP, "no HIP device");
+
+  osy_fluid* fl = new (std::nothrow) osy_fluid();
+  if (!fl) return set_error(OSY_E_NOMEM, "host allocation failed");
+  fl->cfg = *cfg;
+  fl->W = cfg->width;
+  fl->H = cfg->height;
+  fl->P = (int)align_up((size_t)cfg->width, 64);
+  fl->cs = (int)cfg->cell_size;
+  fl->storage = cfg->storage;
+  fl->elem = cfg->storage == OSY_STORAGE_F32 ? 4 : 2;
+  hipError_t e = hipGetDevice(&fl->device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&fl->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete fl;
+    return set_error(OSY_E_HIP, std::string("stream: ") + hipGetErrorString(e));
+  }
+
+  const size_t cells = (size_t)fl->P * fl->H;
+  const size_t fld = align_up(cells * fl->elem, 256);
+  const size_t f32 = align_up(cells * 4, 256);
+  const size_t ref = align_up((size_t)fl->W * fl->H * 4, 256);
+  const size_t flg = align_up(cells, 256);
+  const size_t red = align_up(sizeof(float) * (2 * kReduceBlocks + 2), 256);
+  const size_t dred = align_up(sizeof(double) * (2 * kReduceBlocks + 2), 256);
+  fl->block_bytes = 6 * fld + f32 + ref + flg + red + dred;
+  e = hipMalloc(&fl->block, fl->block_bytes);
+  if (e != hipSuccess) {
+    delete fl;
+    return set_error(OSY_E_NOMEM, std::string("device allocation of ") +
+                                      std::to_string(fl->block_bytes) +
+                                      " bytes failed: " + hipGetErrorString(e));
+  }
+  char* b = (char*)fl->block;
+  for (int k = 0; k < 2; k++) {
+    fl->f.u[k] = b; b += fld;
+    fl->f.v[k] = b; b += fld;
+    fl->f.smoke[k] = b; b += fld;
+  }
+  fl->f.p = (float*)b; b += f32;
+  fl->staging = (float*)b; b += ref;
+  fl->f.flags = (uint8_t*)b; b += flg;
+  fl->f.scratch = (float*)b; b += red;
+  fl->f.dscratch = (double*)b; b += dred;
+  fl->f.prange = fl->f.scratch + 2 * kReduceBlocks;
+
+  e = hipMemsetAsync(fl->block, 0, fl->block_bytes, fl->stream);
+  if (e == hipSuccess) {
+    MaskParams m{};
+    m.W = fl->W; m.H = fl->H; m.P = fl->P;
+    m.enable_drain = cfg->enable_drain;
+    m.obstacle_enable = cfg->obstacle_enable;
+    m.cx = (double)cfg->obstacle_center_x;
+    m.cy = (double)cfg->obstacle_center_y;
+    m.radius = (double)cfg->obstacle_radius;
+    m.pipe_length = cfg->pipe_length;
+    m.pipe_height = cfg->pipe_height;
+    launch_build_flags(m, fl->f.flags, fl->stream);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(fl->stream);
+  if (e != hipSuccess) {
+    delete fl;
+    return set_error(OSY_E_HIP, std::string("init: ") + hipGetErrorString(e));
+  }
+  *out = fl;
+  return OSY_OK;
+}
+
+int enqueue_eager(osy_fluid* fl, const osy_source* src, float d_t, hipStream_t s) {
+  StepParams sp = fl->params(d_t, src);
+  enqueue_step(fl->storage, sp, fl->f, &fl->uv, &fl->sm, fl->projection_kernel, s);
+  OSY_HIP_CHECK(hipGetLastError());
+  return OSY_OK;
+}
+
+int enqueue_graph_steps(osy_fluid* fl, int32_t steps, float d_t, hipStream_t s) {
+  for (int32_t k = 0; k < steps; k++) {
+    osy_fluid::Graph& g = fl->graphs[fl->uv][fl->sm];
+    if (!g.exec || g.d_t != d_t) {
+      if (g.exec) {
+        OSY_HIP_CHECK(hipGraphExecDestroy(g.exec));
+        g.exec = nullptr;
+      }
+      // Capture one step starting from the current buffer parity.  Capture
+      // runs on the handle's stream; the recorded graph launches anywhere.
+      int uv = fl->uv, sm = fl->sm;
+      hipGraph_t graph = nullptr;
+      OSY_HIP_CHECK(hipStreamBeginCapture(fl->stream, hipStreamCaptureModeThreadLocal));
+      StepParams sp = fl->params(d_t, nullptr);
+      enqueue_step(fl->storage, sp, fl->f, &uv, &sm, fl->projection_kernel, fl->stream);
+      hipError_t le = hipGetLastError();
+      hipError_t ce = hipStreamEndCapture(fl->stream, &graph);
+      if (le != hipSuccess || ce != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return set_error(OSY_E_HIP, std::string("graph capture: ") +
+                                        hipGetErrorString(le != hipSuccess ? le : ce));
+      }
+      hipError_t ie = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      if (ie != hipSuccess) {
+        g.exec = nullptr;
+        return set_error(OSY_E_HIP, std::string("graph instantiate: ") + hipGetErrorString(ie));
+      }
+      g.d_t = d_t;
+      g.uv_out = uv;
+      g.sm_out = sm;
+    }
+    OSY_HIP_CHECK(hipGraphLaunch(g.exec, s));
+    fl->uv = g.uv_out;
+    fl->sm = g.sm_out;
+  }
+  return OSY_OK;
+}
+
+int field_elem_ok(int field) { return field >= OSY_FIELD_VEL_X && field <= OSY_FIELD_TOTAL_S; }
+
+}  // namespace
+
+extern "C" {
+
+int osy_abi_version(void) { return OSY_ABI_VERSION; }
+
+const char* osy_last_error(void) { return g_last_error.c_str(); }
+
+int osy_config_defaults(osy_config* cfg) {
+  if (!cfg) return set_error(OSY_E_INVALID, "null config");
+  config_defaults(cfg, 1920, 1080);
+  return OSY_OK;
+}
+
+int osy_config_parse_file(const char* path, osy_config* cfg) {
+  if (!path || !cfg) return set_error(OSY_E_INVALID, "null argument");
+  return config_parse_file(path, cfg);
+}
+
+int osy_config_parse_string(const char* text, osy_config* cfg) {
+  if (!text || !cfg) return set_error(OSY_E_INVALID, "null argument");
+  return config_parse_text(std::string(text), cfg);
+}
+
+int osy_fluid_create(const osy_config* cfg, osy_fluid** out) {
+  try {
+    return create_impl(cfg, out);
+  } catch (const std::exception& ex) {
+    return set_error(OSY_E_NOMEM, ex.what());
+  }
+}
+
+int osy_fluid_create_from_json(const char* path, osy_fluid** out) {
+  osy_config c;
+  int st = osy_config_parse_file(path, &c);
+  if (st != OSY_OK) return st;
+  return osy_fluid_create(&c, out);
+}
+
+int osy_fluid_destroy(osy_fluid* fluid) {
+  if (!fluid) return OSY_OK;
+  (void)hipStreamSynchronize(fluid->stream);
+  delete fluid;
+  return OSY_OK;
+}
+
+int osy_fluid_update(osy_fluid* fl, const osy_source* src, float d_t) {
+  if (!fl) return set_error(OSY_E_INVALID, "null handle");
+  if (!std::isfinite(d_t)) return set_error(OSY_E_INVALID, "d_t must be finite");
+  int st = (src && src->active) ? enqueue_eager(fl, src, d_t, fl->stream)
+                                : enqueue_graph_steps(fl, 1, d_t, fl->stream);
+  if (st != OSY_OK) return st;
+  OSY_HIP_CHECK(hipStreamSynchronize(fl->stream));  // src/fluid.cu:794
+  return OSY_OK;
+}
+
+int osy_fluid_step(osy_fluid* fl, int32_t steps, float d_t) {
+  int st = osy_fluid_step_async(fl, steps, d_t, nullptr);
+  if (st != OSY_OK) return st;
+  OSY_HIP_CHECK(hipStreamSynchronize(fl->stream));
+  return OSY_OK;
+}
+
+int osy_fluid_step_async(osy_fluid* fl, int32_t steps, float d_t, void* hip_stream) {
+  if (!fl) return set_error(OSY_E_INVALID, "null handle");
+  if (steps < 0) return set_error(OSY_E_INVALID, "steps must be >= 0");
+  if (!std::isfinite(d_t)) return set_error(OSY_E_INVALID, "d_t must be finite");
+  hipStream_t s = hip_stream ? (hipStream_t)hip_stream : fl->stream;
+  return enqueue_graph_steps(fl, steps, d_t, s);
+}
+
+int osy_fluid_synchronize(osy_fluid* fl) {
+  if (!fl) return set_error(OSY_E_INVALID, "null handle");
+  OSY_HIP_CHECK(hipStreamSynchronize(fl->stream));
+  return OSY_OK;
+}
+
+int osy_fluid_project(osy_fluid* fl, int32_t sweeps, void* hip_stream) {
+  if (!fl) return set_error(OSY_E_INVALID, "null handle");
+  if (sweeps < 0) return set_error(OSY_E_INVALID, "sweeps must be >= 0");
+  hipStream_t s = hip_stream ? (hipStream_t)hip_stream : fl->stream;
+  StepParams sp = fl->params(fl->cfg.d_t, nullptr);
+  sp.n = sweeps;
+  enqueue_projection_only(fl->storage, sp, fl->f, &fl->uv, fl->projection_kernel, s);
+  OSY_HIP_CHECK(hipGetLastError());
+  return OSY_OK;
+}
+
+int osy_fluid_get_field(osy_fluid* fl, int32_t field, void* dst, size_t bytes) {
+  if (!fl || !dst) return set_error(OSY_E_INVALID, "null argument");
+  if (!field_elem_ok(field)) return set_error(OSY_E_INVALID, "unknown field");
+  if (bytes != (size_t)fl->W * fl->H * 4)
+    return set_error(OSY_E_INVALID, "buffer must hold width*height 4-byte elements");
+  StepParams sp = fl->params(fl->cfg.d_t, nullptr);
+  enqueue_export(field, fl->storage, sp, fl->f, fl->uv, fl->sm, fl->staging, fl->stream);
+  OSY_HIP_CHECK(hipGetLastError());
+  OSY_HIP_CHECK(hipMemcpyAsync(dst, fl->staging, bytes, hipMemcpyDeviceToHost, fl->stream));
+  OSY_HIP_CHECK(hipStreamSynchronize(fl->stream));
+  return OSY_OK;
+}
+
+int osy_fluid_set_field(osy_fluid* fl, int32_t field, const void* src, size_t bytes) {
+  if (!fl || !src) return set_error(OSY_E_INVALID, "null argument");
+  if (field == OSY_FIELD_IS_SOLID || field == OSY_FIELD_TOTAL_S)
+    return set_error(OSY_E_INVALID, "is_solid and total_s are derived from the config; read-only");
+  if (!field_elem_ok(field)) return set_error(OSY_E_INVALID, "unknown field");
+  if (bytes != (size_t)fl->W * fl->H * 4)
+    return set_error(OSY_E_INVALID, "buffer must hold width*height 4-byte elements");
+  OSY_HIP_CHECK(hipMemcpyAsync(fl->staging, src, bytes, hipMemcpyHostToDevice, fl->stream));
+  StepParams sp = fl->params(fl->cfg.d_t, nullptr);
+  enqueue_import(field, fl->storage, sp, fl->f, fl->uv, fl->sm, fl->staging, fl->stream);
+  OSY_HIP_CHECK(hipGetLastError());
+  OSY_HIP_CHECK(hipStreamSynchronize(fl->stream));
+  return OSY_OK;
+}
+
+int osy_fluid_pressure_range(osy_fluid* fl, float* min_p, float* max_p) {
+  if (!fl || !min_p || !max_p) return set_error(OSY_E_INVALID, "null argument");
+  float r[2] = {0.0f, 0.0f};
+  if (fl->cfg.enable_pressure) {
+    OSY_HIP_CHECK(hipMemcpyAsync(r, fl->f.prange, sizeof(r), hipMemcpyDeviceToHost, fl->stream));
+    OSY_HIP_CHECK(hipStreamSynchronize(fl->stream));
+  }
+  *min_p = r[0];
+  *max_p = r[1];
+  return OSY_OK;
+}
+
+int osy_fluid_divergence_norm(osy_fluid* fl, double* l2, double* linf) {
+  if (!fl || !l2 || !linf) return set_error(OSY_E_INVALID, "null argument");
+  StepParams sp = fl->params(fl->cfg.d_t, nullptr);
+  enqueue_divergence(fl->storage, sp, fl->f, fl->uv, fl->stream);
+  OSY_HIP_CHECK(hipGetLastError());
+  double r[2];
+  OSY_HIP_CHECK(hipMemcpyAsync(r, fl->f.dscratch + 2 * kReduceBlocks, sizeof(r),
+                               hipMemcpyDeviceToHost, fl->stream));
+  OSY_HIP_CHECK(hipStreamSynchronize(fl->stream));
+  *l2 = r[0];
+  *linf = r[1];
+  return OSY_OK;
+}
+
+int osy_fluid_get_info(const osy_fluid* fl, osy_fluid_info* info) {
+  if (!fl || !info) return set_error(OSY_E_INVALID, "null argument");
+  info->width = fl->W;
+  info->height = fl->H;
+  info->cell_size = fl->cs;
+  info->projection_n = fl->cfg.projection_n;
+  info->pitch = fl->P;
+  info->storage = fl->storage;
+  info->projection_kernel = PROJ_COLOUR;
+  info->sweeps_per_launch = 1;
+  info->device_bytes = fl->block_bytes;
+  return OSY_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,82 @@
+// kernels.hpp - launch interface of the solver's HIP kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#include "../../include/opensayal.h"
+
+namespace osy {
+
+// Geometry of the solid mask (src/fluid.cu:111-126).
+struct MaskParams {
+  int W, H, P;
+  int enable_drain;
+  int obstacle_enable;
+  double cx, cy, radius;  // radius as the float config value, widened
+  int pipe_length, pipe_height;
+};
+
+// Everything a step needs besides the field pointers.
+struct StepParams {
+  int W, H, P, cs;
+  int n;                 // projection sweeps
+  float o, density, d_t;
+  // apply_external_forces_at (src/fluid.cu:308-349)
+  int smoke_length;
+  int band_lo, band_hi;  // u = speed for band_lo <= j <= band_hi
+  int smoke_count, smoke_height, smoke_mod, smoke_top;  // stripe rule
+  int sband_lo, sband_hi;                                // single-band rule
+  float speed, wind_smoke;
+  int drag_on;
+  float damping;          // expf(-drag_coeff * d_t), computed once on the host
+  float gdt;              // g * d_t
+  int src_active, src_x, src_y;
+  float src_smoke, src_velocity;
+  int pressure_on;
+  int smoke_on;           // enable_smoke && wind_tunnel.smoke != 0
+  int decay_on;
+  float decay_rd;         // decay_rate * d_t
+};
+
+// Device buffers of one simulation; T is the storage type of u, v, smoke.
+struct FieldPtrs {
+  void* u[2];
+  void* v[2];
+  void* smoke[2];
+  float* p;
+  uint8_t* flags;
+  float* scratch;   // reductions: partial results
+  float* prange;    // [min, max] of pressure
+  double* dscratch; // divergence reduction partials
+};
+
+// Number of float slots `scratch` must hold, and doubles for `dscratch`.
+constexpr int kReduceBlocks = 512;
+
+void launch_build_flags(const MaskParams& m, uint8_t* flags, hipStream_t s);
+
+// One full step (src/fluid.cu:770-795).  `uv`/`sm` give the buffer index
+// currently holding u,v / smoke and are advanced to the index holding the
+// step's result.
+void enqueue_step(int storage, const StepParams& sp, const FieldPtrs& f,
+                  int* uv, int* sm, int projection_kernel, hipStream_t s);
+
+// The projection alone (`sp.n` sweeps); advances *uv like enqueue_step.
+void enqueue_projection_only(int storage, const StepParams& sp, const FieldPtrs& f, int* uv,
+                             int projection_kernel, hipStream_t s);
+
+// Divergence L2 / max-abs in FP64 over interior fluid cells; result in
+// dscratch[0], dscratch[1] after the stream reaches this point.
+void enqueue_divergence(int storage, const StepParams& sp, const FieldPtrs& f,
+                        int uv, hipStream_t s);
+
+// Layout conversion between the device layout and the reference's indx
+// order, for a field of storage type `storage` (or float/int32 for the
+// other fields).  `ref` is a device buffer of W*H 4-byte elements.
+void enqueue_export(int field, int storage, const StepParams& sp,
+                    const FieldPtrs& f, int uv, int sm, void* ref, hipStream_t s);
+void enqueue_import(int field, int storage, const StepParams& sp,
+                    const FieldPtrs& f, int uv, int sm, const void* ref, hipStream_t s);
+
+}  // namespace osy

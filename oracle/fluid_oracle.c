@@ -1,0 +1,189 @@
+/*
+ * fluid_oracle.c - CPU oracle for the OpenSayal per-step fluid solver.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker that the HIP solver
+ * is compared against, and the host-core baseline the benchmark times beside
+ * it.  The product library (opensayal_amd/csrc) neither links nor calls it.
+ *
+ * It is a plain-C restatement of the reference's update rule, src/fluid.cu of
+ * gopmur/OpenSayal, with the operation order of that file.  The reference is
+ * CUDA and cannot be built here, and it has no tests or golden data; each
+ * function below is pinned instead by the checks in tests/test_oracle_*.py
+ * (closed forms, invariants, brute force on tiny grids).
+ *
+ * Build: oracle/Makefile (gcc -O2 -fopenmp -ffp-contract=off; no fast-math).
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "../include/opensayal.h"
+
+typedef struct orc_grid {
+  int W, H, cs;
+  const int32_t* solid;   /* is_solid, reference layout */
+  const int32_t* total_s; /* total_s, reference layout  */
+} orc_grid;
+
+/* Fluid::indx (src/fluid.cu:163-165). */
+static inline int orc_idx(const orc_grid* g, int i, int j) {
+  return (g->H - 1 - j) * g->W + i;
+}
+
+/* Fluid::is_valid_fluid = index_is_valid && !is_solid (src/fluid.cu:356-362). */
+static inline int orc_valid(const orc_grid* g, int i, int j) {
+  return i >= 0 && j >= 0 && i < g->W && j < g->H && !g->solid[orc_idx(g, i, j)];
+}
+
+/* ---- single precision: the reference's arithmetic ---------------------- */
+#define REAL float
+#define SFX _f32
+#define ORC_SQRT sqrtf
+#define ORC_EXP expf
+#define ORC_FMIN fminf
+#define ORC_FMAX fmaxf
+#include "fluid_oracle_impl.h"
+#undef REAL
+#undef SFX
+#undef ORC_SQRT
+#undef ORC_EXP
+#undef ORC_FMIN
+#undef ORC_FMAX
+
+/* ---- double precision: the FP64 run for precision-mode error reports --- */
+#define REAL double
+#define SFX _f64
+#define ORC_SQRT sqrt
+#define ORC_EXP exp
+#define ORC_FMIN fmin
+#define ORC_FMAX fmax
+#include "fluid_oracle_impl.h"
+#undef REAL
+#undef SFX
+#undef ORC_SQRT
+#undef ORC_EXP
+#undef ORC_FMIN
+#undef ORC_FMAX
+
+/* Fluid::init_device_memory mask build (src/fluid.cu:111-142).  is_solid is
+ * the left column, bottom and top rows, the right column when the drain is
+ * off, the obstacle disc (distance computed in double with std::pow/sqrt, a
+ * strict `<` against the float radius), and the pipe walls (pipe_length is
+ * fixed at 0 by the reference's parser, so they never appear).  total_s
+ * counts in-bounds non-solid 4-neighbours of every cell; the reference
+ * increments an uninitialised malloc buffer, the intent (and the value for
+ * large grids, where glibc maps zeroed pages) is a count from 0. */
+int orc_build_masks(const osy_config* c, int32_t* is_solid, int32_t* total_s) {
+  const int W = c->width, H = c->height;
+  if (W < 3 || H < 3) return -1;
+  orc_grid g = {W, H, 1, is_solid, total_s};
+  for (int i = 0; i < W; i++) {
+    for (int j = 0; j < H; j++) {
+      const double dx = (double)i - (double)c->obstacle_center_x;
+      const double dy = (double)j - (double)c->obstacle_center_y;
+      const int in_obstacle =
+          c->obstacle_enable && sqrt(dx * dx + dy * dy) < (double)c->obstacle_radius;
+      const int pipe_wall =
+          i < c->pipe_length &&
+          (j == H / 2 - c->pipe_height / 2 - 1 || j == H / 2 + c->pipe_height / 2 + 1);
+      is_solid[orc_idx(&g, i, j)] =
+          (i == 0 || j == 0 || j == H - 1 || (!c->enable_drain && i == W - 1) ||
+           in_obstacle || pipe_wall);
+    }
+  }
+  for (int i = 0; i < W; i++) {
+    for (int j = 0; j < H; j++) {
+      int s = 0;
+      if (i - 1 >= 0 && !is_solid[orc_idx(&g, i - 1, j)]) s++;
+      if (i + 1 < W && !is_solid[orc_idx(&g, i + 1, j)]) s++;
+      if (j - 1 >= 0 && !is_solid[orc_idx(&g, i, j - 1)]) s++;
+      if (j + 1 < H && !is_solid[orc_idx(&g, i, j + 1)]) s++;
+      total_s[orc_idx(&g, i, j)] = s;
+    }
+  }
+  return 0;
+}
+
+/* ---- flat entry points for ctypes -------------------------------------- */
+
+#define ORC_GRID(c, solid, ts) \
+  orc_grid g = {(c)->width, (c)->height, (int)(c)->cell_size, (solid), (ts)}
+
+#define ORC_EXPORTS(R, S)                                                      \
+  void orc_step##S(const osy_config* c, const int32_t* solid,                  \
+                   const int32_t* ts, R* u, R* v, R* smoke, R* p, R* scratch,  \
+                   R* prange, const osy_source* src, R d_t, int steps,         \
+                   int nthreads) {                                             \
+    ORC_GRID(c, solid, ts);                                                    \
+    for (int s = 0; s < steps; s++)                                            \
+      orc_update##S(c, &g, u, v, smoke, p, scratch, prange, src, d_t,          \
+                    nthreads);                                                 \
+  }                                                                            \
+  void orc_phase_forces##S(const osy_config* c, const int32_t* solid,          \
+                           const int32_t* ts, R* u, R* v, R* smoke, R* p,      \
+                           const osy_source* src, R d_t) {                     \
+    ORC_GRID(c, solid, ts);                                                    \
+    orc_forces##S(c, &g, u, v, smoke, p, src, d_t, 1);                         \
+  }                                                                            \
+  void orc_phase_projection##S(const osy_config* c, const int32_t* solid,      \
+                               const int32_t* ts, R* u, R* v, R* p, R d_t,     \
+                               int sweeps, int nthreads) {                     \
+    ORC_GRID(c, solid, ts);                                                    \
+    orc_projection##S(c, &g, u, v, p, d_t, sweeps, nthreads);                  \
+  }                                                                            \
+  void orc_phase_colour##S(const osy_config* c, const int32_t* solid,          \
+                           const int32_t* ts, R* u, R* v, R* p, R d_t,         \
+                           int colour) {                                       \
+    ORC_GRID(c, solid, ts);                                                    \
+    orc_projection_phase##S(c, &g, u, v, p, d_t, colour, 1);                   \
+  }                                                                            \
+  void orc_phase_extrapolation##S(const osy_config* c, const int32_t* solid,   \
+                                  const int32_t* ts, R* u, R* v) {             \
+    ORC_GRID(c, solid, ts);                                                    \
+    orc_extrapolation##S(&g, u, v);                                            \
+  }                                                                            \
+  void orc_phase_advect_velocity##S(const osy_config* c, const int32_t* solid, \
+                                    const int32_t* ts, const R* u,             \
+                                    const R* v, R* u_out, R* v_out, R d_t) {   \
+    ORC_GRID(c, solid, ts);                                                    \
+    orc_advect_velocity##S(&g, u, v, u_out, v_out, d_t, 1);                    \
+  }                                                                            \
+  void orc_phase_advect_smoke##S(const osy_config* c, const int32_t* solid,    \
+                                 const int32_t* ts, const R* u, const R* v,    \
+                                 const R* smoke, R* smoke_out, R d_t) {        \
+    ORC_GRID(c, solid, ts);                                                    \
+    orc_advect_smoke##S(&g, u, v, smoke, smoke_out, d_t, 1);                   \
+  }                                                                            \
+  void orc_phase_decay##S(const osy_config* c, const int32_t* solid,           \
+                          const int32_t* ts, R* smoke, R rate, R d_t) {        \
+    ORC_GRID(c, solid, ts);                                                    \
+    orc_decay##S(&g, smoke, rate, d_t, 1);                                     \
+  }                                                                            \
+  R orc_point_gx##S(const osy_config* c, const int32_t* solid,                 \
+                    const int32_t* ts, const R* u, R x, R y) {                 \
+    ORC_GRID(c, solid, ts);                                                    \
+    return orc_gx##S(&g, u, x, y);                                             \
+  }                                                                            \
+  R orc_point_gy##S(const osy_config* c, const int32_t* solid,                 \
+                    const int32_t* ts, const R* v, R x, R y) {                 \
+    ORC_GRID(c, solid, ts);                                                    \
+    return orc_gy##S(&g, v, x, y);                                             \
+  }                                                                            \
+  R orc_point_idw##S(const osy_config* c, const int32_t* solid,                \
+                     const int32_t* ts, const R* smoke, R x, R y) {            \
+    ORC_GRID(c, solid, ts);                                                    \
+    return orc_idw##S(&g, smoke, x, y);                                        \
+  }                                                                            \
+  void orc_divergence##S(const osy_config* c, const int32_t* solid,            \
+                         const int32_t* ts, const R* u, const R* v,            \
+                         double* out) {                                        \
+    ORC_GRID(c, solid, ts);                                                    \
+    orc_divergence_norm##S(&g, u, v, out);                                     \
+  }
+
+ORC_EXPORTS(float, _f32)
+ORC_EXPORTS(double, _f64)
+
+/* Lets a loader check that it opened this library and not another. */
+int orc_abi_marker(void) { return 0x05a7a1; }
