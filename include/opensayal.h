@@ -128,6 +128,16 @@ typedef struct osy_fluid_info {
   size_t device_bytes;  /* device memory held by this handle               */
 } osy_fluid_info;
 
+/* Launch shape of the projection, this implementation's own knob (the
+ * reference's thread.cuda.block_size_x/y has no counterpart here).  Results
+ * are bit-identical for every valid shape; only speed changes. */
+typedef struct osy_tuning {
+  int32_t projection_kernel; /* 0: one launch per colour phase; 1: tiled    */
+  int32_t region_pairs;      /* tiled: LDS region width in cell pairs       */
+  int32_t region_rows;       /* tiled: LDS region height in cells           */
+  int32_t sweeps_per_launch; /* tiled: sweeps per launch (halo = 2x this)   */
+} osy_tuning;
+
 typedef struct osy_fluid osy_fluid;
 
 /* ---- configuration: replaces ConfigParser (src/config_parser.cpp:11-184) */
@@ -188,6 +198,11 @@ int osy_fluid_pressure_range(osy_fluid* fluid, float* min_pressure,
 int osy_fluid_divergence_norm(osy_fluid* fluid, double* l2, double* linf);
 
 int osy_fluid_get_info(const osy_fluid* fluid, osy_fluid_info* info);
+
+/* Select the projection launch shape (OSY_E_INVALID if the shape does not
+ * fit the kernel's limits).  Captured step graphs are rebuilt on next use. */
+int osy_fluid_set_tuning(osy_fluid* fluid, const osy_tuning* tuning);
+int osy_fluid_get_tuning(const osy_fluid* fluid, osy_tuning* tuning);
 
 /* Message of the calling thread's last failed call ("" if none). */
 const char* osy_last_error(void);

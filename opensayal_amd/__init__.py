@@ -22,7 +22,7 @@ from typing import Optional
 import numpy as np
 
 from ._abi import (FIELDS, STORAGE, OpenSayalError, OsyConfig, OsyFluidInfo, OsySource,
-                   check, lib)
+                   OsyTuning, check, lib)
 
 __all__ = ["Config", "ConfigParser", "Fluid", "Source", "OpenSayalError", "default_config",
            "FIELDS", "STORAGE"]
@@ -86,6 +86,19 @@ class Fluid:
         self.device_bytes = info.device_bytes
         self.projection_kernel = info.projection_kernel
         self.sweeps_per_launch = info.sweeps_per_launch
+
+    @property
+    def tuning(self) -> dict:
+        t = OsyTuning()
+        check(lib().osy_fluid_get_tuning(self._h, ctypes.byref(t)))
+        return {name: getattr(t, name) for name, _ in t._fields_}
+
+    def set_tuning(self, projection_kernel: int = 1, region_pairs: int = 64,
+                   region_rows: int = 64, sweeps_per_launch: int = 4) -> None:
+        """Projection launch shape; results are bit-identical for every shape."""
+        t = OsyTuning(projection_kernel, region_pairs, region_rows, sweeps_per_launch)
+        check(lib().osy_fluid_set_tuning(self._h, ctypes.byref(t)))
+        self._refresh_info()
 
     @property
     def projection_kernel_name(self) -> str:

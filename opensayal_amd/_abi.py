@@ -72,6 +72,13 @@ class OsySource(ctypes.Structure):
                 ("position_x", _i32), ("position_y", _i32)]
 
 
+class OsyTuning(ctypes.Structure):
+    """struct osy_tuning: projection launch shape."""
+
+    _fields_ = [("projection_kernel", _i32), ("region_pairs", _i32), ("region_rows", _i32),
+                ("sweeps_per_launch", _i32)]
+
+
 class OsyFluidInfo(ctypes.Structure):
     _fields_ = [("width", _i32), ("height", _i32), ("cell_size", _i32),
                 ("projection_n", _i32), ("pitch", _i32), ("storage", _i32),
@@ -99,6 +106,8 @@ PROTOTYPES = {
     "osy_fluid_pressure_range": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(_f32), ctypes.POINTER(_f32)]),
     "osy_fluid_divergence_norm": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
     "osy_fluid_get_info": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(OsyFluidInfo)]),
+    "osy_fluid_set_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(OsyTuning)]),
+    "osy_fluid_get_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(OsyTuning)]),
 }
 
 _lib = None

@@ -82,7 +82,7 @@ struct osy_fluid {
   void* block = nullptr;     // one device allocation holding every buffer
   size_t block_bytes = 0;
   float* staging = nullptr;  // W*H*4 bytes, reference-order copies
-  int projection_kernel = PROJ_AUTO;
+  ProjTuning tuning;  // projection kernel shape, chosen at create
   struct Graph {
     hipGraphExec_t exec = nullptr;
     float d_t = 0.0f;
@@ -199,7 +199,8 @@ int create_impl(const osy_config* cfg, osy_fluid** out) {
   fl->f.dscratch = (double*)b; b += dred;
   fl->f.prange = fl->f.scratch + 2 * kReduceBlocks;
 
-  e = hipMemsetAsync(fl->block, 0, fl->block_bytes, fl->stream);
+  e = init_projection_kernels();
+  if (e == hipSuccess) e = hipMemsetAsync(fl->block, 0, fl->block_bytes, fl->stream);
   if (e == hipSuccess) {
     MaskParams m{};
     m.W = fl->W; m.H = fl->H; m.P = fl->P;
@@ -224,7 +225,7 @@ int create_impl(const osy_config* cfg, osy_fluid** out) {
 
 int enqueue_eager(osy_fluid* fl, const osy_source* src, float d_t, hipStream_t s) {
   StepParams sp = fl->params(d_t, src);
-  enqueue_step(fl->storage, sp, fl->f, &fl->uv, &fl->sm, fl->projection_kernel, s);
+  enqueue_step(fl->storage, sp, fl->f, &fl->uv, &fl->sm, fl->tuning, s);
   OSY_HIP_CHECK(hipGetLastError());
   return OSY_OK;
 }
@@ -243,7 +244,7 @@ int enqueue_graph_steps(osy_fluid* fl, int32_t steps, float d_t, hipStream_t s) 
       hipGraph_t graph = nullptr;
       OSY_HIP_CHECK(hipStreamBeginCapture(fl->stream, hipStreamCaptureModeThreadLocal));
       StepParams sp = fl->params(d_t, nullptr);
-      enqueue_step(fl->storage, sp, fl->f, &uv, &sm, fl->projection_kernel, fl->stream);
+      enqueue_step(fl->storage, sp, fl->f, &uv, &sm, fl->tuning, fl->stream);
       hipError_t le = hipGetLastError();
       hipError_t ce = hipStreamEndCapture(fl->stream, &graph);
       if (le != hipSuccess || ce != hipSuccess) {
@@ -353,7 +354,7 @@ int osy_fluid_project(osy_fluid* fl, int32_t sweeps, void* hip_stream) {
   hipStream_t s = hip_stream ? (hipStream_t)hip_stream : fl->stream;
   StepParams sp = fl->params(fl->cfg.d_t, nullptr);
   sp.n = sweeps;
-  enqueue_projection_only(fl->storage, sp, fl->f, &fl->uv, fl->projection_kernel, s);
+  enqueue_projection_only(fl->storage, sp, fl->f, &fl->uv, fl->tuning, s);
   OSY_HIP_CHECK(hipGetLastError());
   return OSY_OK;
 }
@@ -412,6 +413,42 @@ int osy_fluid_divergence_norm(osy_fluid* fl, double* l2, double* linf) {
   return OSY_OK;
 }
 
+int osy_fluid_set_tuning(osy_fluid* fl, const osy_tuning* t) {
+  if (!fl || !t) return set_error(OSY_E_INVALID, "null argument");
+  ProjTuning pt;
+  pt.kernel = t->projection_kernel;
+  pt.pw = t->region_pairs;
+  pt.rh = t->region_rows;
+  pt.k = t->sweeps_per_launch;
+  if (pt.kernel == PROJ_COLOUR) {
+    pt = ProjTuning();
+    pt.kernel = PROJ_COLOUR;
+  } else if (!pt.valid()) {
+    return set_error(OSY_E_INVALID,
+                     "tiled projection needs k >= 1, a tile of at least 2x1 cells, at most " +
+                         std::to_string(kTileMaxItems * kTileThreads) +
+                         " cell pairs per region and at most 160 KiB of LDS");
+  }
+  OSY_HIP_CHECK(hipStreamSynchronize(fl->stream));
+  for (auto& row : fl->graphs)
+    for (auto& g : row)
+      if (g.exec) {
+        OSY_HIP_CHECK(hipGraphExecDestroy(g.exec));
+        g.exec = nullptr;
+      }
+  fl->tuning = pt;
+  return OSY_OK;
+}
+
+int osy_fluid_get_tuning(const osy_fluid* fl, osy_tuning* t) {
+  if (!fl || !t) return set_error(OSY_E_INVALID, "null argument");
+  t->projection_kernel = fl->tuning.kernel;
+  t->region_pairs = fl->tuning.pw;
+  t->region_rows = fl->tuning.rh;
+  t->sweeps_per_launch = fl->tuning.k;
+  return OSY_OK;
+}
+
 int osy_fluid_get_info(const osy_fluid* fl, osy_fluid_info* info) {
   if (!fl || !info) return set_error(OSY_E_INVALID, "null argument");
   info->width = fl->W;
@@ -420,8 +457,8 @@ int osy_fluid_get_info(const osy_fluid* fl, osy_fluid_info* info) {
   info->projection_n = fl->cfg.projection_n;
   info->pitch = fl->P;
   info->storage = fl->storage;
-  info->projection_kernel = PROJ_COLOUR;
-  info->sweeps_per_launch = 1;
+  info->projection_kernel = fl->tuning.kernel;
+  info->sweeps_per_launch = fl->tuning.kernel == PROJ_TILED ? fl->tuning.k : 1;
   info->device_bytes = fl->block_bytes;
   return OSY_OK;
 }

@@ -51,6 +51,43 @@ struct FieldPtrs {
   double* dscratch; // divergence reduction partials
 };
 
+// Projection kernels selectable per handle.
+enum ProjectionKernel : int {
+  PROJ_COLOUR = 0,  // one launch per colour phase, 2n launches, in place
+  PROJ_TILED = 1,   // LDS tiles, several sweeps per launch, ping-pong
+};
+
+// Maximum pair-items one thread of the tiled kernel holds in registers.
+constexpr int kTileMaxItems = 24;
+constexpr int kTileThreads = 256;
+
+// Shape of the tiled projection.  A workgroup loads a region of
+// (2*pw) x rh cells into LDS, runs up to k sweeps on it, and writes back the
+// tile it owns: the region minus a 2k-cell halo on every side, so the tile
+// is (2*pw - 4k) x (rh - 4k) cells.
+struct ProjTuning {
+  int kernel = PROJ_TILED;
+  int pw = 64;  // cell pairs per region row
+  int rh = 64;  // region rows
+  int k = 4;    // sweeps per launch
+  int tile_w() const { return 2 * pw - 4 * k; }
+  int tile_h() const { return rh - 4 * k; }
+  size_t lds_bytes() const {
+    return (size_t)rh * 2 * (pw + 1) * 4 * 2 + (size_t)(rh + 1) * (2 * pw + 1);
+  }
+  bool valid() const {
+    if (kernel == PROJ_COLOUR) return true;
+    return kernel == PROJ_TILED && k >= 1 && pw >= 8 && rh >= 8 && tile_w() >= 2 &&
+           tile_h() >= 1 && (tile_w() % 2) == 0 &&
+           (pw * rh + kTileThreads - 1) / kTileThreads <= kTileMaxItems &&
+           lds_bytes() <= 160 * 1024;
+  }
+  // projection launches for `sweeps` sweeps
+  int launches(int sweeps) const {
+    return kernel == PROJ_COLOUR ? 2 * sweeps : (sweeps + k - 1) / k;
+  }
+};
+
 // Number of float slots `scratch` must hold, and doubles for `dscratch`.
 constexpr int kReduceBlocks = 512;
 
@@ -60,11 +97,11 @@ void launch_build_flags(const MaskParams& m, uint8_t* flags, hipStream_t s);
 // currently holding u,v / smoke and are advanced to the index holding the
 // step's result.
 void enqueue_step(int storage, const StepParams& sp, const FieldPtrs& f,
-                  int* uv, int* sm, int projection_kernel, hipStream_t s);
+                  int* uv, int* sm, const ProjTuning& t, hipStream_t s);
 
 // The projection alone (`sp.n` sweeps); advances *uv like enqueue_step.
 void enqueue_projection_only(int storage, const StepParams& sp, const FieldPtrs& f, int* uv,
-                             int projection_kernel, hipStream_t s);
+                             const ProjTuning& t, hipStream_t s);
 
 // Divergence L2 / max-abs in FP64 over interior fluid cells; result in
 // dscratch[0], dscratch[1] after the stream reaches this point.

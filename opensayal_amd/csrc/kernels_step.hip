@@ -332,7 +332,7 @@ __global__ void k_layout(int mode, int W, int H, int P, T* dev, float* pdev, con
 }
 
 template <typename T>
-void step_t(const StepParams& sp, const FieldPtrs& f, int* uv, int* sm, int projection_kernel,
+void step_t(const StepParams& sp, const FieldPtrs& f, int* uv, int* sm, const ProjTuning& tuning,
             hipStream_t s) {
   const GridView g{sp.W, sp.H, sp.P, sp.cs, f.flags};
   T* u = (T*)f.u[*uv];
@@ -341,7 +341,7 @@ void step_t(const StepParams& sp, const FieldPtrs& f, int* uv, int* sm, int proj
   const dim3 grid = cell_grid(sp.W, sp.H), block(kBX, kBY);
   hipLaunchKernelGGL(k_forces<T>, grid, block, 0, s, sp, u, v, smk, f.p);
   int cur = *uv;
-  enqueue_projection<T>(sp, g, f, &cur, projection_kernel, s);
+  enqueue_projection<T>(sp, g, f, &cur, tuning, s);
   u = (T*)f.u[cur];
   v = (T*)f.v[cur];
   if (sp.pressure_on) {
@@ -395,23 +395,23 @@ void launch_build_flags(const MaskParams& m, uint8_t* flags, hipStream_t s) {
 }
 
 void enqueue_step(int storage, const StepParams& sp, const FieldPtrs& f, int* uv, int* sm,
-                  int projection_kernel, hipStream_t s) {
+                  const ProjTuning& tuning, hipStream_t s) {
   switch (storage) {
-    case OSY_STORAGE_F16: step_t<__half>(sp, f, uv, sm, projection_kernel, s); break;
-    case OSY_STORAGE_BF16: step_t<__hip_bfloat16>(sp, f, uv, sm, projection_kernel, s); break;
-    default: step_t<float>(sp, f, uv, sm, projection_kernel, s); break;
+    case OSY_STORAGE_F16: step_t<__half>(sp, f, uv, sm, tuning, s); break;
+    case OSY_STORAGE_BF16: step_t<__hip_bfloat16>(sp, f, uv, sm, tuning, s); break;
+    default: step_t<float>(sp, f, uv, sm, tuning, s); break;
   }
 }
 
 void enqueue_projection_only(int storage, const StepParams& sp, const FieldPtrs& f, int* uv,
-                             int projection_kernel, hipStream_t s) {
+                             const ProjTuning& tuning, hipStream_t s) {
   const GridView g{sp.W, sp.H, sp.P, sp.cs, f.flags};
   switch (storage) {
-    case OSY_STORAGE_F16: enqueue_projection<__half>(sp, g, f, uv, projection_kernel, s); break;
+    case OSY_STORAGE_F16: enqueue_projection<__half>(sp, g, f, uv, tuning, s); break;
     case OSY_STORAGE_BF16:
-      enqueue_projection<__hip_bfloat16>(sp, g, f, uv, projection_kernel, s);
+      enqueue_projection<__hip_bfloat16>(sp, g, f, uv, tuning, s);
       break;
-    default: enqueue_projection<float>(sp, g, f, uv, projection_kernel, s); break;
+    default: enqueue_projection<float>(sp, g, f, uv, tuning, s); break;
   }
 }
 
