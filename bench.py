@@ -74,6 +74,32 @@ def cpu_baseline(cfg, seed: int, budget_s: float = 12.0) -> dict:
     }
 
 
+def sweep_shapes(fl, stream, handle, steps: int = 40) -> None:
+    """Per-step time for a grid of projection launch shapes (one JSON line each)."""
+    import torch
+
+    shapes = [dict(projection_kernel=0)]
+    for pw in (32, 48, 64, 96):
+        for rh in (32, 48, 64, 96, 128):
+            for k in (2, 3, 4, 6, 8):
+                shapes.append(dict(projection_kernel=1, region_pairs=pw, region_rows=rh,
+                                   sweeps_per_launch=k))
+    for sh in shapes:
+        try:
+            fl.set_tuning(**sh)
+        except Exception:
+            continue
+        fl.step_async(3, stream=handle)
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        fl.step_async(steps, stream=handle)
+        b.record(stream)
+        torch.cuda.synchronize()
+        ms = a.elapsed_time(b) / steps
+        print(json.dumps({"shape": sh, "ms_per_step": round(ms, 4),
+                          "gcell_steps_s": round(fl.width * fl.height / ms / 1e6, 3)}), flush=True)
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -83,6 +109,8 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--kernel-reps", type=int, default=20)
+    ap.add_argument("--sweep", action="store_true",
+                    help="time a grid of projection launch shapes instead")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -104,6 +132,11 @@ def main() -> int:
 
     stream = torch.cuda.Stream()
     handle = stream.cuda_stream
+
+    if args.sweep:
+        sweep_shapes(fl, stream, handle)
+        return 0
+
     fl.step_async(args.warmup, stream=handle)
     torch.cuda.synchronize()
 

@@ -191,6 +191,55 @@ def test_half_storage_tracks_fp64(storage):
         assert err < bound, (name, err)
 
 
+TUNINGS = [
+    dict(projection_kernel=0),
+    dict(projection_kernel=1, region_pairs=64, region_rows=64, sweeps_per_launch=4),
+    dict(projection_kernel=1, region_pairs=32, region_rows=24, sweeps_per_launch=2),
+    dict(projection_kernel=1, region_pairs=48, region_rows=40, sweeps_per_launch=3),
+    dict(projection_kernel=1, region_pairs=96, region_rows=64, sweeps_per_launch=8),
+    dict(projection_kernel=1, region_pairs=16, region_rows=12, sweeps_per_launch=1),
+]
+
+
+@pytest.mark.parametrize("t", range(len(TUNINGS)))
+@pytest.mark.parametrize("pressure", [0, 1])
+def test_projection_shapes_bit_exact(t, pressure):
+    """Every launch shape gives the oracle's bits (n = 13 is not a multiple
+    of any k above; the 77x53 grid clips tiles on both edges)."""
+    cfg = make_config(77, 53, projection_n=13, enable_pressure=pressure, wind_speed=2.0,
+                      g=-5.0, obstacle_center_x=30, obstacle_center_y=20, obstacle_radius=8.5)
+    fl, o = pair(cfg, seed=12)
+    fl.set_tuning(**TUNINGS[t])
+    if pressure:
+        p0 = np.random.default_rng(3).uniform(-1, 1, (53, 77)).astype(np.float32)
+        fl.set_field("pressure", p0)
+        o.set_state(p=p0)
+    fl.project()
+    o.projection()
+    assert_bitwise(fl, o, f"projection {TUNINGS[t]}")
+    fl.step(3)
+    o.update(steps=3)
+    assert_bitwise(fl, o, f"steps {TUNINGS[t]}")
+
+
+@pytest.mark.parametrize("t", [1, 4])
+def test_full_size_tiled_steps_bit_exact(t):
+    cfg = wl.plain_full()
+    fl, o = pair(cfg, seed=13)
+    fl.set_tuning(**TUNINGS[t])
+    fl.step(3)
+    o.update(steps=3)
+    assert_bitwise(fl, o, f"full {TUNINGS[t]}")
+
+
+def test_invalid_tuning_rejected():
+    fl = Fluid(make_config(64, 36))
+    with pytest.raises(OpenSayalError):
+        fl.set_tuning(projection_kernel=1, region_pairs=8, region_rows=8, sweeps_per_launch=4)
+    with pytest.raises(OpenSayalError):
+        fl.set_tuning(projection_kernel=1, region_pairs=200, region_rows=200, sweeps_per_launch=2)
+
+
 def test_errors_are_reported():
     with pytest.raises(OpenSayalError) as e:
         Fluid(make_config(64, 36, viscosity=0.001))
