@@ -79,10 +79,10 @@ def sweep_shapes(fl, stream, handle, steps: int = 40) -> None:
     import torch
 
     shapes = [dict(projection_kernel=0)]
-    for pw in (32, 48, 64, 96):
-        for rh in (32, 48, 64, 96, 128):
+    for tw in (112, 176, 240, 368, 496):
+        for th in (8, 16, 24, 32, 48, 64):
             for k in (2, 3, 4, 6, 8):
-                shapes.append(dict(projection_kernel=1, region_pairs=pw, region_rows=rh,
+                shapes.append(dict(projection_kernel=1, tile_width=tw, tile_height=th,
                                    sweeps_per_launch=k))
     for sh in shapes:
         try:

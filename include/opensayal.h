@@ -133,8 +133,8 @@ typedef struct osy_fluid_info {
  * are bit-identical for every valid shape; only speed changes. */
 typedef struct osy_tuning {
   int32_t projection_kernel; /* 0: one launch per colour phase; 1: tiled    */
-  int32_t region_pairs;      /* tiled: LDS region width in cell pairs       */
-  int32_t region_rows;       /* tiled: LDS region height in cells           */
+  int32_t tile_width;        /* tiled: cells a workgroup owns along i (even) */
+  int32_t tile_height;       /* tiled: cells a workgroup owns along j       */
   int32_t sweeps_per_launch; /* tiled: sweeps per launch (halo = 2x this)   */
 } osy_tuning;
 

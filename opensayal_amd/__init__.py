@@ -93,10 +93,10 @@ class Fluid:
         check(lib().osy_fluid_get_tuning(self._h, ctypes.byref(t)))
         return {name: getattr(t, name) for name, _ in t._fields_}
 
-    def set_tuning(self, projection_kernel: int = 1, region_pairs: int = 64,
-                   region_rows: int = 64, sweeps_per_launch: int = 4) -> None:
+    def set_tuning(self, projection_kernel: int = 1, tile_width: int = 240,
+                   tile_height: int = 32, sweeps_per_launch: int = 4) -> None:
         """Projection launch shape; results are bit-identical for every shape."""
-        t = OsyTuning(projection_kernel, region_pairs, region_rows, sweeps_per_launch)
+        t = OsyTuning(projection_kernel, tile_width, tile_height, sweeps_per_launch)
         check(lib().osy_fluid_set_tuning(self._h, ctypes.byref(t)))
         self._refresh_info()
 

@@ -75,7 +75,7 @@ class OsySource(ctypes.Structure):
 class OsyTuning(ctypes.Structure):
     """struct osy_tuning: projection launch shape."""
 
-    _fields_ = [("projection_kernel", _i32), ("region_pairs", _i32), ("region_rows", _i32),
+    _fields_ = [("projection_kernel", _i32), ("tile_width", _i32), ("tile_height", _i32),
                 ("sweeps_per_launch", _i32)]
 
 

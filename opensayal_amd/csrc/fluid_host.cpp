@@ -146,6 +146,20 @@ namespace {
 
 size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
+// Starting launch shape before any autotuning: 4 sweeps per launch, tiles
+// up to 240 cells wide (a 256-cell region, two wavefronts per row) and as
+// tall as the LDS allows, never wider than the grid.
+ProjTuning default_tuning(int W, bool pressure) {
+  ProjTuning t;
+  t.kernel = PROJ_TILED;
+  t.k = 4;
+  t.tw = W >= 240 ? 240 : (W + 1) / 2 * 2;
+  t.th = 32;
+  while (t.th > 4 && !t.valid(pressure)) t.th -= 4;
+  if (!t.valid(pressure)) t.kernel = PROJ_COLOUR;
+  return t;
+}
+
 int create_impl(const osy_config* cfg, osy_fluid** out) {
   if (!cfg || !out) return set_error(OSY_E_INVALID, "null argument");
   *out = nullptr;
@@ -164,6 +178,7 @@ int create_impl(const osy_config* cfg, osy_fluid** out) {
   fl->cs = (int)cfg->cell_size;
   fl->storage = cfg->storage;
   fl->elem = cfg->storage == OSY_STORAGE_F32 ? 4 : 2;
+  fl->tuning = default_tuning(fl->W, cfg->enable_pressure != 0);
   hipError_t e = hipGetDevice(&fl->device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&fl->stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
@@ -417,17 +432,18 @@ int osy_fluid_set_tuning(osy_fluid* fl, const osy_tuning* t) {
   if (!fl || !t) return set_error(OSY_E_INVALID, "null argument");
   ProjTuning pt;
   pt.kernel = t->projection_kernel;
-  pt.pw = t->region_pairs;
-  pt.rh = t->region_rows;
+  pt.tw = t->tile_width;
+  pt.th = t->tile_height;
   pt.k = t->sweeps_per_launch;
   if (pt.kernel == PROJ_COLOUR) {
     pt = ProjTuning();
     pt.kernel = PROJ_COLOUR;
-  } else if (!pt.valid()) {
+  } else if (!pt.valid(fl->cfg.enable_pressure != 0)) {
     return set_error(OSY_E_INVALID,
-                     "tiled projection needs k >= 1, a tile of at least 2x1 cells, at most " +
-                         std::to_string(kTileMaxItems * kTileThreads) +
-                         " cell pairs per region and at most 160 KiB of LDS");
+                     "tiled projection needs 1 <= k <= 64, an even tile width >= 2, a tile "
+                     "height >= 1, a region of at most " +
+                         std::to_string(2 * 64 * kTileWaves) +
+                         " cells per row and at most 160 KiB of LDS");
   }
   OSY_HIP_CHECK(hipStreamSynchronize(fl->stream));
   for (auto& row : fl->graphs)
@@ -443,8 +459,8 @@ int osy_fluid_set_tuning(osy_fluid* fl, const osy_tuning* t) {
 int osy_fluid_get_tuning(const osy_fluid* fl, osy_tuning* t) {
   if (!fl || !t) return set_error(OSY_E_INVALID, "null argument");
   t->projection_kernel = fl->tuning.kernel;
-  t->region_pairs = fl->tuning.pw;
-  t->region_rows = fl->tuning.rh;
+  t->tile_width = fl->tuning.tw;
+  t->tile_height = fl->tuning.th;
   t->sweeps_per_launch = fl->tuning.k;
   return OSY_OK;
 }

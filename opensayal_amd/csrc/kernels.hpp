@@ -57,30 +57,38 @@ enum ProjectionKernel : int {
   PROJ_TILED = 1,   // LDS tiles, several sweeps per launch, ping-pong
 };
 
-// Maximum pair-items one thread of the tiled kernel holds in registers.
-constexpr int kTileMaxItems = 24;
-constexpr int kTileThreads = 256;
+// Workgroup of the tiled kernel: 8 wavefronts of 64 lanes.
+constexpr int kTileWaves = 8;
+constexpr int kTileThreads = 64 * kTileWaves;
+constexpr size_t kLdsLimit = 160 * 1024;
 
-// Shape of the tiled projection.  A workgroup loads a region of
-// (2*pw) x rh cells into LDS, runs up to k sweeps on it, and writes back the
-// tile it owns: the region minus a 2k-cell halo on every side, so the tile
-// is (2*pw - 4k) x (rh - 4k) cells.
+// Shape of the tiled projection.  A workgroup owns a tile of tw x th cells;
+// it loads the tile plus a 2k-cell halo on every side (the region) into LDS,
+// runs up to k sweeps there and writes the tile back.  A lane handles one
+// pair of horizontally adjacent cells; a region row of pw pairs takes
+// ceil(pw / 64) wavefronts.
 struct ProjTuning {
   int kernel = PROJ_TILED;
-  int pw = 64;  // cell pairs per region row
-  int rh = 64;  // region rows
-  int k = 4;    // sweeps per launch
-  int tile_w() const { return 2 * pw - 4 * k; }
-  int tile_h() const { return rh - 4 * k; }
-  size_t lds_bytes() const {
-    return (size_t)rh * 2 * (pw + 1) * 4 * 2 + (size_t)(rh + 1) * (2 * pw + 1);
+  int tw = 240;  // tile width (even)
+  int th = 32;   // tile height
+  int k = 4;     // sweeps per launch
+  int region_w() const { return tw + 4 * k; }
+  int region_h() const { return th + 4 * k; }
+  int pw() const { return region_w() / 2; }
+  int waves_per_row() const {
+    const int n = (pw() + 63) / 64;
+    return n <= 1 ? 1 : n <= 2 ? 2 : n <= 4 ? 4 : 8;
   }
-  bool valid() const {
+  // LDS: u and v (float), gate and flag bytes, and pressure sums (float)
+  size_t lds_bytes(bool pressure) const {
+    const size_t cells = (size_t)region_h() * 2 * (pw() + 1);
+    const size_t flag_bytes = (size_t)(region_h() + 1) * (region_w() + 1);
+    return cells * 4 * (pressure ? 3 : 2) + cells + flag_bytes;
+  }
+  bool valid(bool pressure) const {
     if (kernel == PROJ_COLOUR) return true;
-    return kernel == PROJ_TILED && k >= 1 && pw >= 8 && rh >= 8 && tile_w() >= 2 &&
-           tile_h() >= 1 && (tile_w() % 2) == 0 &&
-           (pw * rh + kTileThreads - 1) / kTileThreads <= kTileMaxItems &&
-           lds_bytes() <= 160 * 1024;
+    return kernel == PROJ_TILED && k >= 1 && k <= 64 && tw >= 2 && th >= 1 && tw % 2 == 0 &&
+           pw() <= 64 * kTileWaves && lds_bytes(pressure) <= kLdsLimit;
   }
   // projection launches for `sweeps` sweeps
   int launches(int sweeps) const {

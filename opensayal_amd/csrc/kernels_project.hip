@@ -7,16 +7,16 @@
 //
 // Two kernels:
 //  * k_project_colour: one launch per colour phase, in place (2n launches);
-//    kept for tiny grids and as the simple restatement.
-//  * k_project_tiled: temporal blocking.  A workgroup loads a region of
-//    u, v and cell flags into LDS, runs up to k sweeps there, and writes the
-//    tile it owns to the other half of a ping-pong pair.  A colour phase
+//    the plain restatement, kept for tiny grids.
+//  * k_project_tiled: temporal blocking.  A workgroup loads its tile plus a
+//    halo of u, v and cell flags into LDS, runs up to k sweeps there, and
+//    writes the tile to the other half of a ping-pong pair.  A colour phase
 //    changes only faces of the cells it updates, so a value can be wrong
 //    after phase s only within s cells of the region edge; a halo of 2k
-//    cells therefore keeps every owned face exact through 2k phases.  Each
-//    phase updates only the cells whose results still reach the owned tile
-//    (a box that shrinks by one cell per phase), so the redundant halo work
-//    falls off as the sweeps proceed.
+//    cells keeps every face of the tile exact through 2k phases.  Each phase
+//    updates only the cells whose results still reach the tile (a box that
+//    shrinks by one cell per phase), so the redundant halo work falls off as
+//    the sweeps proceed.
 #include "projection.hpp"
 
 #pragma clang fp contract(off)
@@ -71,15 +71,50 @@ enum : uint32_t {
 };
 
 struct TiledArgs {
-  int pw, rh, k;      // region: 2*pw x rh cells, k sweeps per launch
-  int tw, th;         // owned tile
-  int ntx;            // tiles per row
-  int sweeps;         // sweeps this launch (<= k)
+  int tw, th, k;  // owned tile, sweeps per launch (halo 2k)
+  int pw, rh;     // region: pw cell pairs (2*pw cells) x rh rows
+  int wpr;        // wavefronts per region row
+  int ntx;        // tiles per grid row
+  int sweeps;     // sweeps this launch (<= k)
 };
 
-// LDS layout: U and V hold rh rows of two half-rows each, the even columns
-// of the region row then the odd ones, so the 64 lanes of a wave that update
-// one colour of a row touch 64 consecutive words (no bank conflicts).
+// Two horizontally adjacent elements (cells 2q, 2q+1 of a row).
+template <typename T> struct PairOf;
+template <> struct PairOf<float> { using type = float2; };
+template <> struct PairOf<__half> { using type = __half2; };
+template <> struct PairOf<__hip_bfloat16> { using type = __hip_bfloat162; };
+
+template <typename T>
+__device__ inline void load_pair(const T* __restrict__ a, long k, bool in0, bool in1, float& x0,
+                                 float& x1) {
+  if (in0 && in1) {  // k is even and rows start on 64-element boundaries
+    const typename PairOf<T>::type pr = *reinterpret_cast<const typename PairOf<T>::type*>(a + k);
+    x0 = Store<T>::ld(pr.x);
+    x1 = Store<T>::ld(pr.y);
+  } else {
+    x0 = in0 ? ldf(a, k) : 0.0f;
+    x1 = in1 ? ldf(a, k + 1) : 0.0f;
+  }
+}
+
+template <typename T>
+__device__ inline void store_pair(T* __restrict__ a, long k, bool in0, bool in1, float x0,
+                                  float x1) {
+  if (in0 && in1) {
+    typename PairOf<T>::type pr;
+    pr.x = Store<T>::st(x0);
+    pr.y = Store<T>::st(x1);
+    *reinterpret_cast<typename PairOf<T>::type*>(a + k) = pr;
+  } else {
+    if (in0) stf(a, k, x0);
+    if (in1) stf(a, k + 1, x1);
+  }
+}
+
+// LDS layout: U, V (and the pressure sums S) hold rh rows of two half-rows,
+// the even columns of the region row then the odd ones, so the lanes that
+// update one colour of a row touch consecutive words.  Lane l of column
+// block cb handles cell pair q = 64*cb + l of every row its wavefront visits.
 template <typename T, bool PRESSURE>
 __global__ void __launch_bounds__(kTileThreads)
     k_project_tiled(StepParams sp, const uint8_t* __restrict__ flags, const T* __restrict__ u_in,
@@ -88,111 +123,114 @@ __global__ void __launch_bounds__(kTileThreads)
   extern __shared__ __align__(16) char smem[];
   const int pw = a.pw, rh = a.rh, K = a.k;
   const int pwp = pw + 1;
-  const int rs = 2 * pwp;      // LDS row stride (floats)
-  const int rw = 2 * pw;       // region width in cells
-  const int fs = rw + 1;       // flag row stride (bytes)
+  const int rs = 2 * pwp;  // LDS row stride (elements)
+  const int rw = 2 * pw;   // region width (cells)
+  const int fs = rw + 1;   // flag row stride (bytes)
+  const int ncell = rh * rs;
   float* U = (float*)smem;
-  float* V = U + rh * rs;
-  uint8_t* F = (uint8_t*)(V + rh * rs);
+  float* V = U + ncell;
+  float* S = V + ncell;  // pressure sums (PRESSURE only)
+  uint8_t* G = (uint8_t*)(PRESSURE ? S + ncell : V + ncell);
+  uint8_t* F = G + ncell;
+
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int cb = wave % a.wpr;             // column block of this wavefront
+  const int r0 = wave / a.wpr;             // its first row
+  const int rstep = kTileWaves / a.wpr;    // its row stride
+  const int q = cb * 64 + lane;            // this lane's cell pair
+  const bool lane_on = q < pw;
 
   const int tx = blockIdx.x % a.ntx, ty = blockIdx.x / a.ntx;
   const int x0 = tx * a.tw, y0 = ty * a.th;
-  const int gx0 = x0 - 2 * K, gy0 = y0 - 2 * K;  // region origin (even: tw, 2K even)
+  const int gx0 = x0 - 2 * K, gy0 = y0 - 2 * K;  // region origin; gx0 is even
   const int W = sp.W, H = sp.H, P = sp.P;
-  const int nitems = pw * rh;
-  const int M = (nitems + kTileThreads - 1) / kTileThreads;
 
-  // flags of the region plus one extra column and row (right/top neighbours)
-  for (int q = threadIdx.x; q < (rh + 1) * fs; q += kTileThreads) {
-    const int r = q / fs, c = q - r * fs;
-    const int gi = gx0 + c, gj = gy0 + r;
-    F[q] = ((unsigned)gi < (unsigned)W && (unsigned)gj < (unsigned)H) ? flags[(long)gj * P + gi]
-                                                                      : (uint8_t)0;
-  }
-  // u, v of the region, de-interleaved by column parity
-  for (int q = threadIdx.x; q < nitems; q += kTileThreads) {
-    const int r = q / pw, pp = q - r * pw;
+  // flags of the region plus one column right and one row above
+  for (int r = r0; r <= rh; r += rstep) {
     const int gj = gy0 + r;
     const bool row_in = (unsigned)gj < (unsigned)H;
+    if (lane_on) {
+#pragma unroll
+      for (int b = 0; b < 2; b++) {
+        const int gi = gx0 + 2 * q + b;
+        F[r * fs + 2 * q + b] =
+            (row_in && (unsigned)gi < (unsigned)W) ? flags[(long)gj * P + gi] : (uint8_t)0;
+      }
+    }
+  }
+  for (int r = threadIdx.x; r <= rh; r += kTileThreads) {
+    const int gi = gx0 + rw, gj = gy0 + r;
+    F[r * fs + rw] = ((unsigned)gi < (unsigned)W && (unsigned)gj < (unsigned)H)
+                         ? flags[(long)gj * P + gi]
+                         : (uint8_t)0;
+  }
+  // u, v (and pressure) of the region
+  for (int r = r0; r < rh; r += rstep) {
+    if (!lane_on) break;
+    const int gj = gy0 + r;
+    const int gi = gx0 + 2 * q;
+    const bool row_in = (unsigned)gj < (unsigned)H;
+    const bool in0 = row_in && (unsigned)gi < (unsigned)W;
+    const bool in1 = row_in && (unsigned)(gi + 1) < (unsigned)W;
+    const long gk = (long)gj * P + gi;
+    float a0, a1;
+    const int l0 = r * rs + q, l1 = l0 + pwp;
+    load_pair(u_in, gk, in0, in1, a0, a1);
+    U[l0] = a0;
+    U[l1] = a1;
+    load_pair(v_in, gk, in0, in1, a0, a1);
+    V[l0] = a0;
+    V[l1] = a1;
+    if constexpr (PRESSURE) {
+      S[l0] = in0 ? p[gk] : 0.0f;
+      S[l1] = in1 ? p[gk + 1] : 0.0f;
+    }
+  }
+  __syncthreads();
+  // gate bytes
+  for (int r = r0; r < rh; r += rstep) {
+    if (!lane_on) break;
 #pragma unroll
     for (int b = 0; b < 2; b++) {
-      const int gi = gx0 + 2 * pp + b;
-      const bool in = row_in && (unsigned)gi < (unsigned)W;
-      const long gk = (long)gj * P + gi;
-      U[r * rs + b * pwp + pp] = in ? ldf(u_in, gk) : 0.0f;
-      V[r * rs + b * pwp + pp] = in ? ldf(v_in, gk) : 0.0f;
+      const int lc = 2 * q + b;
+      const uint8_t f = F[r * fs + lc];
+      const uint8_t fr = F[r * fs + lc + 1];
+      const uint8_t ft = F[(r + 1) * fs + lc];
+      uint32_t g = 0;
+      if (f & CF_UPD) g |= G_UPD;
+      if (f & CF_FU) g |= G_L;
+      if (fr & CF_FU) g |= G_R;
+      if (f & CF_FV) g |= G_B;
+      if (ft & CF_FV) g |= G_T;
+      g |= (uint32_t)cf_total_s(f) << G_S_SHIFT;
+      G[r * rs + b * pwp + q] = (uint8_t)g;
     }
   }
   __syncthreads();
 
-  // per-item constants: position, gate bytes of the item's two cells, and
-  // the pressure accumulators of owned cells
-  int rp[kTileMaxItems];
-  uint32_t gates[kTileMaxItems];
-  float pacc[PRESSURE ? kTileMaxItems : 1][2];
-#pragma unroll
-  for (int m = 0; m < kTileMaxItems; m++) {
-    rp[m] = 0;
-    gates[m] = 0;
-    if (m < M) {
-      const int q = threadIdx.x + m * kTileThreads;
-      if (q < nitems) {
-        const int r = q / pw, pp = q - r * pw;
-        rp[m] = (r << 16) | pp;
-        uint32_t g2 = 0;
-#pragma unroll
-        for (int b = 0; b < 2; b++) {
-          const int lc = 2 * pp + b;
-          const uint8_t f = F[r * fs + lc];
-          const uint8_t fr = F[r * fs + lc + 1];
-          const uint8_t ft = F[(r + 1) * fs + lc];
-          uint32_t g = 0;
-          if (f & CF_UPD) g |= G_UPD;
-          if (f & CF_FU) g |= G_L;
-          if (fr & CF_FU) g |= G_R;
-          if (f & CF_FV) g |= G_B;
-          if (ft & CF_FV) g |= G_T;
-          g |= (uint32_t)cf_total_s(f) << G_S_SHIFT;
-          g2 |= g << (8 * b);
-          if constexpr (PRESSURE) {
-            const int gi = gx0 + lc, gj = gy0 + r;
-            const bool owned = lc >= 2 * K && lc < 2 * K + a.tw && r >= 2 * K &&
-                               r < 2 * K + a.th && gi < W && gj < H;
-            pacc[m][b] = owned ? p[(long)gj * P + gi] : 0.0f;
-          }
-        }
-        gates[m] = g2;
-      }
-    }
-  }
-
   const float o = sp.o;
   const int nphase = 2 * a.sweeps;
   for (int s = 0; s < nphase; s++) {
-    const int c = s & 1;                 // 0: even cells first, as the reference
-    const int e = nphase - 1 - s;        // phases still to come after this one
-    const int lo = 2 * K - 1 - e;        // needed box, region-local cells
+    const int c = s & 1;           // even cells first, as the reference
+    const int e = nphase - 1 - s;  // phases still to come after this one
+    const int lo = 2 * K - 1 - e;  // box of cells that still reach the tile
     const int chi = 2 * K + a.tw + e;
     const int rhi = 2 * K + a.th + e;
-#pragma unroll
-    for (int m = 0; m < kTileMaxItems; m++) {
-      if (m >= M) break;
-      const uint32_t gg = gates[m];
-      const int r = rp[m] >> 16, pp = rp[m] & 0xffff;
-      const int b = (c + gy0 + r) & 1;   // the pair's cell of colour c (gx0 is even)
-      const int lc = 2 * pp + b;
-      const uint32_t g = (gg >> (8 * b)) & 0xffu;
-      if (!(g & G_UPD) || r < lo || r >= rhi || lc < lo || lc >= chi) continue;
-      const int own = r * rs + b * pwp + pp;
+    int r = r0;
+    if (lo > r0) r += ((lo - r0 + rstep - 1) / rstep) * rstep;
+    for (; r < rhi; r += rstep) {
+      const int b = (c + gy0 + r) & 1;  // the pair's cell of colour c
+      const int lc = 2 * q + b;
+      if (!lane_on || lc < lo || lc >= chi) continue;
+      const int own = r * rs + b * pwp + q;
+      const uint32_t g = G[own];
+      if (!(g & G_UPD)) continue;
       const int right = b ? own - pwp + 1 : own + pwp;  // u(i+1): next pair's even cell
       const int top = own + rs;
       const float uu = U[own], ur = U[right], vv = V[own], vt = V[top];
       const float div = ((ur - uu) + vt) - vv;
       const float d = o * (div / (float)(g >> G_S_SHIFT));
-      if constexpr (PRESSURE) {
-        const float t = ((d * sp.density) * (float)sp.cs) / sp.d_t;
-        pacc[m][b] = pacc[m][b] + t;
-      }
+      if constexpr (PRESSURE) S[own] = S[own] + (((d * sp.density) * (float)sp.cs) / sp.d_t);
       if (g & G_L) U[own] = uu + d;
       if (g & G_R) U[right] = ur - d;
       if (g & G_B) V[own] = vv + d;
@@ -202,24 +240,22 @@ __global__ void __launch_bounds__(kTileThreads)
   }
 
   // write back the owned tile
-#pragma unroll
-  for (int m = 0; m < kTileMaxItems; m++) {
-    if (m >= M) break;
-    const int q = threadIdx.x + m * kTileThreads;
-    if (q >= nitems) break;
-    const int r = rp[m] >> 16, pp = rp[m] & 0xffff;
-    if (r < 2 * K || r >= 2 * K + a.th) continue;
+  for (int r = r0 + ((2 * K - r0 + rstep - 1) / rstep) * rstep; r < 2 * K + a.th; r += rstep) {
+    if (!lane_on) break;
     const int gj = gy0 + r;
-    if (gj >= H) continue;
-#pragma unroll
-    for (int b = 0; b < 2; b++) {
-      const int lc = 2 * pp + b;
-      const int gi = gx0 + lc;
-      if (lc < 2 * K || lc >= 2 * K + a.tw || gi >= W) continue;
-      const long gk = (long)gj * P + gi;
-      stf(u_out, gk, U[r * rs + b * pwp + pp]);
-      stf(v_out, gk, V[r * rs + b * pwp + pp]);
-      if constexpr (PRESSURE) p[gk] = pacc[m][b];
+    if (gj >= H) break;
+    const int lc = 2 * q;
+    const int gi = gx0 + lc;
+    const bool in0 = lc >= 2 * K && lc < 2 * K + a.tw && gi < W;
+    const bool in1 = lc + 1 >= 2 * K && lc + 1 < 2 * K + a.tw && gi + 1 < W;
+    if (!in0 && !in1) continue;
+    const long gk = (long)gj * P + gi;
+    const int l0 = r * rs + q, l1 = l0 + pwp;
+    store_pair(u_out, gk, in0, in1, U[l0], U[l1]);
+    store_pair(v_out, gk, in0, in1, V[l0], V[l1]);
+    if constexpr (PRESSURE) {
+      if (in0) p[gk] = S[l0];
+      if (in1) p[gk + 1] = S[l1];
     }
   }
 }
@@ -228,14 +264,15 @@ template <typename T, bool PRESSURE>
 void launch_tiled(const StepParams& sp, const GridView& g, const FieldPtrs& f, int* cur,
                   const ProjTuning& t, hipStream_t s) {
   TiledArgs a;
-  a.pw = t.pw;
-  a.rh = t.rh;
+  a.tw = t.tw;
+  a.th = t.th;
   a.k = t.k;
-  a.tw = t.tile_w();
-  a.th = t.tile_h();
+  a.pw = t.pw();
+  a.rh = t.region_h();
+  a.wpr = t.waves_per_row();
   a.ntx = (sp.W + a.tw - 1) / a.tw;
   const int nty = (sp.H + a.th - 1) / a.th;
-  const size_t lds = t.lds_bytes();
+  const size_t lds = t.lds_bytes(PRESSURE);
   for (int done = 0; done < sp.n; done += t.k) {
     a.sweeps = (sp.n - done < t.k) ? sp.n - done : t.k;
     const int nxt = 1 - *cur;
@@ -248,7 +285,7 @@ void launch_tiled(const StepParams& sp, const GridView& g, const FieldPtrs& f, i
 
 template <typename T>
 hipError_t allow_large_lds() {
-  const int bytes = 160 * 1024;
+  const int bytes = (int)kLdsLimit;
   hipError_t e = hipFuncSetAttribute((const void*)k_project_tiled<T, false>,
                                      hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
   if (e == hipSuccess)
@@ -269,7 +306,7 @@ hipError_t init_projection_kernels() {
 template <typename T>
 void enqueue_projection(const StepParams& sp, const GridView& g, const FieldPtrs& f, int* cur,
                         const ProjTuning& t, hipStream_t s) {
-  if (t.kernel == PROJ_TILED) {
+  if (t.kernel == PROJ_TILED && t.valid(sp.pressure_on)) {
     if (sp.pressure_on)
       launch_tiled<T, true>(sp, g, f, cur, t, s);
     else

@@ -193,11 +193,12 @@ def test_half_storage_tracks_fp64(storage):
 
 TUNINGS = [
     dict(projection_kernel=0),
-    dict(projection_kernel=1, region_pairs=64, region_rows=64, sweeps_per_launch=4),
-    dict(projection_kernel=1, region_pairs=32, region_rows=24, sweeps_per_launch=2),
-    dict(projection_kernel=1, region_pairs=48, region_rows=40, sweeps_per_launch=3),
-    dict(projection_kernel=1, region_pairs=96, region_rows=64, sweeps_per_launch=8),
-    dict(projection_kernel=1, region_pairs=16, region_rows=12, sweeps_per_launch=1),
+    dict(projection_kernel=1, tile_width=240, tile_height=24, sweeps_per_launch=4),
+    dict(projection_kernel=1, tile_width=56, tile_height=16, sweeps_per_launch=2),
+    dict(projection_kernel=1, tile_width=84, tile_height=28, sweeps_per_launch=3),
+    dict(projection_kernel=1, tile_width=160, tile_height=12, sweeps_per_launch=8),
+    dict(projection_kernel=1, tile_width=2, tile_height=1, sweeps_per_launch=1),
+    dict(projection_kernel=1, tile_width=480, tile_height=8, sweeps_per_launch=2),
 ]
 
 
@@ -235,9 +236,11 @@ def test_full_size_tiled_steps_bit_exact(t):
 def test_invalid_tuning_rejected():
     fl = Fluid(make_config(64, 36))
     with pytest.raises(OpenSayalError):
-        fl.set_tuning(projection_kernel=1, region_pairs=8, region_rows=8, sweeps_per_launch=4)
+        fl.set_tuning(projection_kernel=1, tile_width=15, tile_height=8, sweeps_per_launch=4)
     with pytest.raises(OpenSayalError):
-        fl.set_tuning(projection_kernel=1, region_pairs=200, region_rows=200, sweeps_per_launch=2)
+        fl.set_tuning(projection_kernel=1, tile_width=400, tile_height=200, sweeps_per_launch=2)
+    with pytest.raises(OpenSayalError):
+        fl.set_tuning(projection_kernel=1, tile_width=1100, tile_height=4, sweeps_per_launch=2)
 
 
 def test_errors_are_reported():
