@@ -60,15 +60,41 @@ __global__ void __launch_bounds__(256) k_project_colour(StepParams sp, const uin
 
 // ---- tiled kernel ----------------------------------------------------------
 
-// Per-cell gate byte of the tiled kernel, built in LDS from the flags.
+// Per-cell gate byte of the tiled kernel, built in LDS from the flags.  The
+// four face gates are set only for projected cells, so a cell that is not
+// projected changes nothing when it runs the update.
 enum : uint32_t {
-  G_UPD = 1u << 0,  // cell is projected
-  G_L = 1u << 1,    // u(i,j) moves   (left neighbour fluid)
-  G_R = 1u << 2,    // u(i+1,j) moves (right neighbour fluid)
-  G_B = 1u << 3,    // v(i,j) moves   (bottom neighbour fluid)
-  G_T = 1u << 4,    // v(i,j+1) moves (top neighbour fluid)
-  G_S_SHIFT = 5,    // bits 5..7: total_s
+  G_L = 1u << 0,     // u(i,j) moves   (left neighbour fluid)
+  G_R = 1u << 1,     // u(i+1,j) moves (right neighbour fluid)
+  G_B = 1u << 2,     // v(i,j) moves   (bottom neighbour fluid)
+  G_T = 1u << 3,     // v(i,j+1) moves (top neighbour fluid)
+  G_UPD = 1u << 4,   // cell is projected
+  G_L2_SHIFT = 5,    // bits 5..6: log2(total_s) when total_s is 1, 2 or 4
+  G_SLOW = 1u << 7,  // projected with total_s 0 or 3: divide
 };
+
+// d = o * (div / s) for the cell with gate byte g.  For s = 1, 2, 4 the
+// quotient is div * 2^-log2(s), exact; for s = 0 or 3 (cells against a
+// wall or the obstacle) the lanes that need it divide.  Exhaustively checked
+// against IEEE division over all binary32 inputs (tests/test_quotient.py).
+__device__ inline float relaxed_quotient(float div, uint32_t g, float o) {
+  const float y = __uint_as_float(0x3F800000u - (((g >> G_L2_SHIFT) & 3u) << 23));
+  float q = div * y;
+  if (g & G_SLOW) q = div / (float)__builtin_popcount(g & 15u);
+  return o * q;
+}
+
+// x + gate(d): d when the gate bit is set, else -0 (x + -0 == x for every x,
+// signed zeros and NaN included), so the face keeps its bits when closed.
+__device__ inline float gated_add(float x, float d, uint32_t g, int bit) {
+  const uint32_t m = (uint32_t)((int32_t)(g << (31 - bit)) >> 31);
+  return x + __uint_as_float((__float_as_uint(d) & m) | (~m & 0x80000000u));
+}
+// x - gate(d): d when set, else +0 (x - +0 == x for every x).
+__device__ inline float gated_sub(float x, float d, uint32_t g, int bit) {
+  const uint32_t m = (uint32_t)((int32_t)(g << (31 - bit)) >> 31);
+  return x - __uint_as_float(__float_as_uint(d) & m);
+}
 
 struct TiledArgs {
   int tw, th, k;  // owned tile, sweeps per launch (halo 2k)
@@ -133,7 +159,9 @@ __global__ void __launch_bounds__(kTileThreads)
   uint8_t* G = (uint8_t*)(PRESSURE ? S + ncell : V + ncell);
   uint8_t* F = G + ncell;
 
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  // wavefront-uniform values live in scalar registers
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int cb = wave % a.wpr;             // column block of this wavefront
   const int r0 = wave / a.wpr;             // its first row
   const int rstep = kTileWaves / a.wpr;    // its row stride
@@ -197,12 +225,18 @@ __global__ void __launch_bounds__(kTileThreads)
       const uint8_t fr = F[r * fs + lc + 1];
       const uint8_t ft = F[(r + 1) * fs + lc];
       uint32_t g = 0;
-      if (f & CF_UPD) g |= G_UPD;
-      if (f & CF_FU) g |= G_L;
-      if (fr & CF_FU) g |= G_R;
-      if (f & CF_FV) g |= G_B;
-      if (ft & CF_FV) g |= G_T;
-      g |= (uint32_t)cf_total_s(f) << G_S_SHIFT;
+      if (f & CF_UPD) {
+        g = G_UPD;
+        if (f & CF_FU) g |= G_L;
+        if (fr & CF_FU) g |= G_R;
+        if (f & CF_FV) g |= G_B;
+        if (ft & CF_FV) g |= G_T;
+        const int s = cf_total_s(f);
+        if (s == 0 || s == 3)
+          g |= G_SLOW;
+        else
+          g |= (uint32_t)(s >> 1) << G_L2_SHIFT;
+      }
       G[r * rs + b * pwp + q] = (uint8_t)g;
     }
   }
@@ -218,23 +252,32 @@ __global__ void __launch_bounds__(kTileThreads)
     const int rhi = 2 * K + a.th + e;
     int r = r0;
     if (lo > r0) r += ((lo - r0 + rstep - 1) / rstep) * rstep;
+    // the pair's cell of colour c alternates with the row; b and the offset
+    // of u(i+1) are the same for every lane of a row
+    int b = (c + gy0 + r) & 1;
+    const int bstep = rstep & 1;
+    int own = r * rs + q + b * pwp;
     for (; r < rhi; r += rstep) {
-      const int b = (c + gy0 + r) & 1;  // the pair's cell of colour c
       const int lc = 2 * q + b;
-      if (!lane_on || lc < lo || lc >= chi) continue;
-      const int own = r * rs + b * pwp + q;
-      const uint32_t g = G[own];
-      if (!(g & G_UPD)) continue;
-      const int right = b ? own - pwp + 1 : own + pwp;  // u(i+1): next pair's even cell
-      const int top = own + rs;
-      const float uu = U[own], ur = U[right], vv = V[own], vt = V[top];
-      const float div = ((ur - uu) + vt) - vv;
-      const float d = o * (div / (float)(g >> G_S_SHIFT));
-      if constexpr (PRESSURE) S[own] = S[own] + (((d * sp.density) * (float)sp.cs) / sp.d_t);
-      if (g & G_L) U[own] = uu + d;
-      if (g & G_R) U[right] = ur - d;
-      if (g & G_B) V[own] = vv + d;
-      if (g & G_T) V[top] = vt - d;
+      if (lane_on && lc >= lo && lc < chi) {
+        // every face a cell of this colour touches is touched by no other
+        // cell of the phase, so the four writes are unconditional
+        const int right = b ? own - pwp + 1 : own + pwp;  // u(i+1): next pair's even cell
+        const int top = own + rs;
+        const uint32_t g = G[own];
+        const float uu = U[own], ur = U[right], vv = V[own], vt = V[top];
+        const float div = ((ur - uu) + vt) - vv;
+        const float d = relaxed_quotient(div, g, o);
+        if constexpr (PRESSURE) {
+          if (g & G_UPD) S[own] = S[own] + (((d * sp.density) * (float)sp.cs) / sp.d_t);
+        }
+        U[own] = gated_add(uu, d, g, 0);
+        U[right] = gated_sub(ur, d, g, 1);
+        V[own] = gated_add(vv, d, g, 2);
+        V[top] = gated_sub(vt, d, g, 3);
+      }
+      own += bstep ? (b ? rstep * rs - pwp : rstep * rs + pwp) : rstep * rs;
+      b ^= bstep;
     }
     __syncthreads();
   }
