@@ -79,11 +79,16 @@ def sweep_shapes(fl, stream, handle, steps: int = 40) -> None:
     import torch
 
     shapes = [dict(projection_kernel=0)]
-    for tw in (112, 176, 240, 368, 496):
-        for th in (8, 16, 24, 32, 48, 64):
-            for k in (2, 3, 4, 6, 8):
+    for tw in (112, 176, 240):
+        for th in (16, 24):
+            for k in (3, 4):
                 shapes.append(dict(projection_kernel=1, tile_width=tw, tile_height=th,
                                    sweeps_per_launch=k))
+    for rows in (4, 8, 16):
+        for k in (1, 2, 3, 4, 5, 6, 8, 10, 12):
+            if 8 * rows - 4 * k >= 1:
+                shapes.append(dict(projection_kernel=2, tile_width=128 - 4 * k,
+                                   tile_height=8 * rows - 4 * k, sweeps_per_launch=k))
     for sh in shapes:
         try:
             fl.set_tuning(**sh)

@@ -147,14 +147,14 @@ namespace {
 size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
 // Starting launch shape before any autotuning: 4 sweeps per launch, tiles
-// up to 240 cells wide (a 256-cell region, two wavefronts per row) and as
-// tall as the LDS allows, never wider than the grid.
+// of 112 x 16 cells (a 128 x 32 region: one wavefront per region row, three
+// workgroups per CU), never wider than the grid.
 ProjTuning default_tuning(int W, bool pressure) {
   ProjTuning t;
   t.kernel = PROJ_TILED;
   t.k = 4;
-  t.tw = W >= 240 ? 240 : (W + 1) / 2 * 2;
-  t.th = 32;
+  t.tw = W >= 112 ? 112 : (W + 1) / 2 * 2;
+  t.th = 16;
   while (t.th > 4 && !t.valid(pressure)) t.th -= 4;
   if (!t.valid(pressure)) t.kernel = PROJ_COLOUR;
   return t;

@@ -55,7 +55,13 @@ struct FieldPtrs {
 enum ProjectionKernel : int {
   PROJ_COLOUR = 0,  // one launch per colour phase, 2n launches, in place
   PROJ_TILED = 1,   // LDS tiles, several sweeps per launch, ping-pong
+  PROJ_REGS = 2,    // register-resident tiles, several sweeps per launch
 };
+
+// Register-resident kernel: 8 wavefronts stacked vertically, each holding
+// 128 cells x kRegRows rows in VGPRs; the region is 128 x 8*kRegRows cells.
+constexpr int kRegWaves = 8;
+inline bool reg_rows_supported(int rows) { return rows == 4 || rows == 8 || rows == 16; }
 
 // Workgroup of the tiled kernel: 8 wavefronts of 64 lanes.
 constexpr int kTileWaves = 8;
@@ -85,8 +91,13 @@ struct ProjTuning {
     const size_t flag_bytes = (size_t)(region_h() + 1) * (region_w() + 1);
     return cells * 4 * (pressure ? 3 : 2) + cells + flag_bytes;
   }
+  // PROJ_REGS: rows each wavefront holds
+  int reg_rows() const { return (th + 4 * k) / kRegWaves; }
   bool valid(bool pressure) const {
     if (kernel == PROJ_COLOUR) return true;
+    if (kernel == PROJ_REGS)
+      return k >= 1 && k <= 28 && tw == 128 - 4 * k && th >= 1 &&
+             (th + 4 * k) % kRegWaves == 0 && reg_rows_supported(reg_rows());
     return kernel == PROJ_TILED && k >= 1 && k <= 64 && tw >= 2 && th >= 1 && tw % 2 == 0 &&
            pw() <= 64 * kTileWaves && lds_bytes(pressure) <= kLdsLimit;
   }

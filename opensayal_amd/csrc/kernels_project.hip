@@ -303,6 +303,226 @@ __global__ void __launch_bounds__(kTileThreads)
   }
 }
 
+// ---- register-resident kernel ------------------------------------------------
+//
+// Eight wavefronts stacked vertically; wavefront w holds region rows
+// [w*R, w*R + R), lane l holds cells 2l and 2l+1 of each of them: u, v and
+// the gate byte of both cells, all in VGPRs for the whole launch.  The
+// region is 128 x 8R cells and the owned tile (128 - 4k) x (8R - 4k).
+//
+// In a colour phase each row has one active cell per lane (parity b).  For
+// b = 0 every face the cell touches is in the lane's own registers.  For
+// b = 1 the right face u(2l+2) belongs to lane l+1: it is read and updated
+// through DPP lane shifts.  The top face of a wavefront's last row belongs
+// to the wavefront above: its value is published through LDS before the
+// phase and the update sent back after it (two barriers per phase).
+
+__device__ inline float lane_from_right(float x) {  // lane l gets lane l+1's x (0 for lane 63)
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x130, 0xf, 0xf, true));
+}
+__device__ inline float lane_from_left(float x) {  // lane l gets lane l-1's x (0 for lane 0)
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x138, 0xf, 0xf, true));
+}
+__device__ inline uint32_t lane_from_right_u(uint32_t x) {
+  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x130, 0xf, 0xf, true);
+}
+
+// gate of d: d when the bit is set, else +0
+__device__ inline float gate_pos(float d, uint32_t g, int bit) {
+  const uint32_t m = (uint32_t)((int32_t)(g << (31 - bit)) >> 31);
+  return __uint_as_float(__float_as_uint(d) & m);
+}
+
+// Gate byte of a cell from its flag f, the right neighbour's flag fr and
+// the top neighbour's flag ft (same encoding as the LDS-tiled kernel).
+__device__ inline uint32_t gate_byte(uint32_t f, uint32_t fr, uint32_t ft) {
+  if (!(f & CF_UPD)) return 0;
+  uint32_t g = G_UPD;
+  if (f & CF_FU) g |= G_L;
+  if (fr & CF_FU) g |= G_R;
+  if (f & CF_FV) g |= G_B;
+  if (ft & CF_FV) g |= G_T;
+  const uint32_t s = (f >> CF_S_SHIFT) & 7u;
+  if (s == 0 || s == 3)
+    g |= G_SLOW;
+  else
+    g |= (s >> 1) << G_L2_SHIFT;
+  return g;
+}
+
+struct RegArgs {
+  int tw, th, k;  // owned tile and sweeps per launch (halo 2k)
+  int ntx;        // tiles per grid row
+  int sweeps;     // sweeps this launch (<= k)
+};
+
+__device__ inline bool lc_in_tile(int lc, int K, int tw) { return lc >= 2 * K && lc < 2 * K + tw; }
+
+// update_pressure_at (src/fluid.cu:221-227): ((d * density) * cell_size) / d_t
+__device__ inline float pressure_term(float d, float density, float csf, float d_t) {
+  return ((d * density) * csf) / d_t;
+}
+
+template <int R, int PB, bool PRESSURE>
+__device__ inline void reg_phase_rows(float (&u0)[R], float (&u1)[R], float (&v0)[R],
+                                      float (&v1)[R], const uint32_t (&gw)[R], float (&s0)[R],
+                                      float (&s1)[R], int rbase, int lo, int rhi, uint32_t m0,
+                                      uint32_t m1, float vt0, float vt1, float& dtop, float o,
+                                      float density, float csf, float d_t) {
+#pragma unroll
+  for (int rho = 0; rho < R; rho++) {
+    const int r = rbase + rho;
+    if (r < lo || r >= rhi) continue;  // wavefront-uniform
+    if (((PB + rho) & 1) == 0) {
+      // active cell 2l: faces u0, u1, v0[rho], v0[rho+1]
+      const uint32_t g = gw[rho] & m0;
+      const float uu = u0[rho], ur = u1[rho], vv = v0[rho];
+      const float vt = (rho + 1 < R) ? v0[rho + 1 < R ? rho + 1 : rho] : vt0;
+      const float div = ((ur - uu) + vt) - vv;
+      const float d = relaxed_quotient(div, g, o);
+      if constexpr (PRESSURE) {
+        if (g & G_UPD) s0[rho] = s0[rho] + pressure_term(d, density, csf, d_t);
+      }
+      u0[rho] = gated_add(uu, d, g, 0);
+      u1[rho] = gated_sub(ur, d, g, 1);
+      v0[rho] = gated_add(vv, d, g, 2);
+      if (rho + 1 < R)
+        v0[rho + 1 < R ? rho + 1 : rho] = gated_sub(vt, d, g, 3);
+      else
+        dtop = gate_pos(d, g, 3);
+    } else {
+      // active cell 2l+1: faces u1, u0 of lane l+1, v1[rho], v1[rho+1]
+      const uint32_t g = (gw[rho] >> 8) & m1;
+      const float uu = u1[rho], ur = lane_from_right(u0[rho]), vv = v1[rho];
+      const float vt = (rho + 1 < R) ? v1[rho + 1 < R ? rho + 1 : rho] : vt1;
+      const float div = ((ur - uu) + vt) - vv;
+      const float d = relaxed_quotient(div, g, o);
+      if constexpr (PRESSURE) {
+        if (g & G_UPD) s1[rho] = s1[rho] + pressure_term(d, density, csf, d_t);
+      }
+      u1[rho] = gated_add(uu, d, g, 0);
+      u0[rho] = u0[rho] - lane_from_left(gate_pos(d, g, 1));
+      v1[rho] = gated_add(vv, d, g, 2);
+      if (rho + 1 < R)
+        v1[rho + 1 < R ? rho + 1 : rho] = gated_sub(vt, d, g, 3);
+      else
+        dtop = gate_pos(d, g, 3);
+    }
+  }
+}
+
+template <typename T, bool PRESSURE, int R>
+__global__ void __launch_bounds__(64 * kRegWaves)
+    k_project_regs(StepParams sp, const uint8_t* __restrict__ flags, const T* __restrict__ u_in,
+                   const T* __restrict__ v_in, T* __restrict__ u_out, T* __restrict__ v_out,
+                   float* __restrict__ p, RegArgs a) {
+  __shared__ float2 vrow[kRegWaves][64];  // row 0 (v0, v1) of each wavefront
+  __shared__ float dsend[kRegWaves][64];  // top-row update for the wavefront above
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int K = a.k;
+  const int tx = blockIdx.x % a.ntx, ty = blockIdx.x / a.ntx;
+  const int x0 = tx * a.tw, y0 = ty * a.th;
+  const int gx0 = x0 - 2 * K, gy0 = y0 - 2 * K;  // region origin; gx0 is even
+  const int W = sp.W, H = sp.H, P = sp.P;
+  const int rbase = w * R;  // first region row of this wavefront
+  const int gi = gx0 + 2 * lane;
+  const bool col0 = (unsigned)gi < (unsigned)W, col1 = (unsigned)(gi + 1) < (unsigned)W;
+
+  float u0[R], u1[R], v0[R], v1[R], s0[R], s1[R];
+  uint32_t gw[R];
+  // flags: rows rbase .. rbase+R (one above for the top faces), columns
+  // 2l, 2l+1 of this lane and 2l+2 (lane l+1's first column; lane 63 loads
+  // its own)
+  uint32_t fl0[R + 1], fl1[R + 1], fl2[R + 1];
+#pragma unroll
+  for (int rho = 0; rho <= R; rho++) {
+    const int gj = gy0 + rbase + rho;
+    const bool row_in = (unsigned)gj < (unsigned)H;
+    const long gk = (long)gj * P + gi;
+    fl0[rho] = (row_in && col0) ? flags[gk] : 0u;
+    fl1[rho] = (row_in && col1) ? flags[gk + 1] : 0u;
+    const uint32_t from_right = lane_from_right_u(fl0[rho]);
+    fl2[rho] = (lane == 63) ? ((row_in && (unsigned)(gi + 2) < (unsigned)W) ? flags[gk + 2] : 0u)
+                            : from_right;
+  }
+#pragma unroll
+  for (int rho = 0; rho < R; rho++) {
+    const int gj = gy0 + rbase + rho;
+    const bool row_in = (unsigned)gj < (unsigned)H;
+    const long gk = (long)gj * P + gi;
+    load_pair(u_in, gk, row_in && col0, row_in && col1, u0[rho], u1[rho]);
+    load_pair(v_in, gk, row_in && col0, row_in && col1, v0[rho], v1[rho]);
+    if constexpr (PRESSURE) {
+      s0[rho] = (row_in && col0) ? p[gk] : 0.0f;
+      s1[rho] = (row_in && col1) ? p[gk + 1] : 0.0f;
+    } else {
+      s0[rho] = s1[rho] = 0.0f;
+    }
+    gw[rho] = gate_byte(fl0[rho], fl1[rho], fl0[rho + 1]) |
+              (gate_byte(fl1[rho], fl2[rho], fl1[rho + 1]) << 8);
+  }
+
+  const float o = sp.o;
+  const float csf = (float)sp.cs;
+  const int nphase = 2 * a.sweeps;
+  for (int s = 0; s < nphase; s++) {
+    const int c = s & 1;
+    const int e = nphase - 1 - s;
+    const int lo = 2 * K - 1 - e;
+    const int chi = 2 * K + a.tw + e;
+    const int rhi = 2 * K + a.th + e;
+    const int lc = 2 * lane;
+    const uint32_t m0 = (lc >= lo && lc < chi) ? 0xffu : 0u;
+    const uint32_t m1 = (lc + 1 >= lo && lc + 1 < chi) ? 0xffu : 0u;
+    // publish row 0 for the wavefront below, read the row above
+    if (w > 0) vrow[w][lane] = make_float2(v0[0], v1[0]);
+    __syncthreads();
+    float vt0 = 0.0f, vt1 = 0.0f;
+    if (w < kRegWaves - 1) {
+      const float2 t = vrow[w + 1][lane];
+      vt0 = t.x;
+      vt1 = t.y;
+    }
+    const int pb = (c + gy0 + rbase) & 1;  // parity of row 0's active cell
+    float dtop = 0.0f;
+    if (pb == 0)
+      reg_phase_rows<R, 0, PRESSURE>(u0, u1, v0, v1, gw, s0, s1, rbase, lo, rhi, m0, m1, vt0,
+                                     vt1, dtop, o, sp.density, csf, sp.d_t);
+    else
+      reg_phase_rows<R, 1, PRESSURE>(u0, u1, v0, v1, gw, s0, s1, rbase, lo, rhi, m0, m1, vt0,
+                                     vt1, dtop, o, sp.density, csf, sp.d_t);
+    if (w < kRegWaves - 1) dsend[w][lane] = dtop;
+    __syncthreads();
+    if (w > 0) {
+      // the wavefront below updated the top faces of its last row: those are
+      // this wavefront's row-0 v of the parity opposite to pb
+      const float dd = dsend[w - 1][lane];
+      if (pb == 0)
+        v1[0] = v1[0] - dd;
+      else
+        v0[0] = v0[0] - dd;
+    }
+  }
+
+  // write back the owned tile
+  const bool own0 = lc_in_tile(2 * lane, K, a.tw) && col0;
+  const bool own1 = lc_in_tile(2 * lane + 1, K, a.tw) && col1;
+#pragma unroll
+  for (int rho = 0; rho < R; rho++) {
+    const int r = rbase + rho;
+    const int gj = gy0 + r;
+    if (r < 2 * K || r >= 2 * K + a.th || gj >= H) continue;
+    const long gk = (long)gj * P + gi;
+    store_pair(u_out, gk, own0, own1, u0[rho], u1[rho]);
+    store_pair(v_out, gk, own0, own1, v0[rho], v1[rho]);
+    if constexpr (PRESSURE) {
+      if (own0) p[gk] = s0[rho];
+      if (own1) p[gk + 1] = s1[rho];
+    }
+  }
+}
+
 template <typename T, bool PRESSURE>
 void launch_tiled(const StepParams& sp, const GridView& g, const FieldPtrs& f, int* cur,
                   const ProjTuning& t, hipStream_t s) {
@@ -323,6 +543,35 @@ void launch_tiled(const StepParams& sp, const GridView& g, const FieldPtrs& f, i
                        s, sp, g.flags, (const T*)f.u[*cur], (const T*)f.v[*cur], (T*)f.u[nxt],
                        (T*)f.v[nxt], f.p, a);
     *cur = nxt;
+  }
+}
+
+template <typename T, bool PRESSURE, int R>
+void launch_regs_r(const StepParams& sp, const GridView& g, const FieldPtrs& f, int* cur,
+                   const ProjTuning& t, hipStream_t s) {
+  RegArgs a;
+  a.tw = t.tw;
+  a.th = t.th;
+  a.k = t.k;
+  a.ntx = (sp.W + a.tw - 1) / a.tw;
+  const int nty = (sp.H + a.th - 1) / a.th;
+  for (int done = 0; done < sp.n; done += t.k) {
+    a.sweeps = (sp.n - done < t.k) ? sp.n - done : t.k;
+    const int nxt = 1 - *cur;
+    hipLaunchKernelGGL((k_project_regs<T, PRESSURE, R>), dim3(a.ntx * nty), dim3(64 * kRegWaves),
+                       0, s, sp, g.flags, (const T*)f.u[*cur], (const T*)f.v[*cur], (T*)f.u[nxt],
+                       (T*)f.v[nxt], f.p, a);
+    *cur = nxt;
+  }
+}
+
+template <typename T, bool PRESSURE>
+void launch_regs(const StepParams& sp, const GridView& g, const FieldPtrs& f, int* cur,
+                 const ProjTuning& t, hipStream_t s) {
+  switch (t.reg_rows()) {
+    case 4: launch_regs_r<T, PRESSURE, 4>(sp, g, f, cur, t, s); break;
+    case 8: launch_regs_r<T, PRESSURE, 8>(sp, g, f, cur, t, s); break;
+    default: launch_regs_r<T, PRESSURE, 16>(sp, g, f, cur, t, s); break;
   }
 }
 
@@ -349,6 +598,13 @@ hipError_t init_projection_kernels() {
 template <typename T>
 void enqueue_projection(const StepParams& sp, const GridView& g, const FieldPtrs& f, int* cur,
                         const ProjTuning& t, hipStream_t s) {
+  if (t.kernel == PROJ_REGS && t.valid(sp.pressure_on)) {
+    if (sp.pressure_on)
+      launch_regs<T, true>(sp, g, f, cur, t, s);
+    else
+      launch_regs<T, false>(sp, g, f, cur, t, s);
+    return;
+  }
   if (t.kernel == PROJ_TILED && t.valid(sp.pressure_on)) {
     if (sp.pressure_on)
       launch_tiled<T, true>(sp, g, f, cur, t, s);

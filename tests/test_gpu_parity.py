@@ -199,6 +199,10 @@ TUNINGS = [
     dict(projection_kernel=1, tile_width=160, tile_height=12, sweeps_per_launch=8),
     dict(projection_kernel=1, tile_width=2, tile_height=1, sweeps_per_launch=1),
     dict(projection_kernel=1, tile_width=480, tile_height=8, sweeps_per_launch=2),
+    dict(projection_kernel=2, tile_width=120, tile_height=24, sweeps_per_launch=2),
+    dict(projection_kernel=2, tile_width=112, tile_height=48, sweeps_per_launch=4),
+    dict(projection_kernel=2, tile_width=116, tile_height=116, sweeps_per_launch=3),
+    dict(projection_kernel=2, tile_width=124, tile_height=28, sweeps_per_launch=1),
 ]
 
 
@@ -223,7 +227,7 @@ def test_projection_shapes_bit_exact(t, pressure):
     assert_bitwise(fl, o, f"steps {TUNINGS[t]}")
 
 
-@pytest.mark.parametrize("t", [1, 4])
+@pytest.mark.parametrize("t", [1, 4, 8])
 def test_full_size_tiled_steps_bit_exact(t):
     cfg = wl.plain_full()
     fl, o = pair(cfg, seed=13)
