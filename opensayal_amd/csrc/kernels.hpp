@@ -95,9 +95,10 @@ struct ProjTuning {
   int reg_rows() const { return (th + 4 * k) / kRegWaves; }
   bool valid(bool pressure) const {
     if (kernel == PROJ_COLOUR) return true;
-    if (kernel == PROJ_REGS)
+    if (kernel == PROJ_REGS)  // pressure sums need 2 more VGPRs per row: 16 rows spill
       return k >= 1 && k <= 28 && tw == 128 - 4 * k && th >= 1 &&
-             (th + 4 * k) % kRegWaves == 0 && reg_rows_supported(reg_rows());
+             (th + 4 * k) % kRegWaves == 0 && reg_rows_supported(reg_rows()) &&
+             !(pressure && reg_rows() > 8);
     return kernel == PROJ_TILED && k >= 1 && k <= 64 && tw >= 2 && th >= 1 && tw % 2 == 0 &&
            pw() <= 64 * kTileWaves && lds_bytes(pressure) <= kLdsLimit;
   }

@@ -363,51 +363,111 @@ __device__ inline float pressure_term(float d, float density, float csf, float d
   return ((d * density) * csf) / d_t;
 }
 
-template <int R, int PB, bool PRESSURE>
-__device__ inline void reg_phase_rows(float (&u0)[R], float (&u1)[R], float (&v0)[R],
-                                      float (&v1)[R], const uint32_t (&gw)[R], float (&s0)[R],
-                                      float (&s1)[R], int rbase, int lo, int rhi, uint32_t m0,
-                                      uint32_t m1, float vt0, float vt1, float& dtop, float o,
-                                      float density, float csf, float d_t) {
-#pragma unroll
-  for (int rho = 0; rho < R; rho++) {
-    const int r = rbase + rho;
-    if (r < lo || r >= rhi) continue;  // wavefront-uniform
-    if (((PB + rho) & 1) == 0) {
-      // active cell 2l: faces u0, u1, v0[rho], v0[rho+1]
-      const uint32_t g = gw[rho] & m0;
-      const float uu = u0[rho], ur = u1[rho], vv = v0[rho];
-      const float vt = (rho + 1 < R) ? v0[rho + 1 < R ? rho + 1 : rho] : vt0;
-      const float div = ((ur - uu) + vt) - vv;
-      const float d = relaxed_quotient(div, g, o);
-      if constexpr (PRESSURE) {
-        if (g & G_UPD) s0[rho] = s0[rho] + pressure_term(d, density, csf, d_t);
-      }
-      u0[rho] = gated_add(uu, d, g, 0);
-      u1[rho] = gated_sub(ur, d, g, 1);
-      v0[rho] = gated_add(vv, d, g, 2);
-      if (rho + 1 < R)
-        v0[rho + 1 < R ? rho + 1 : rho] = gated_sub(vt, d, g, 3);
-      else
-        dtop = gate_pos(d, g, 3);
-    } else {
-      // active cell 2l+1: faces u1, u0 of lane l+1, v1[rho], v1[rho+1]
-      const uint32_t g = (gw[rho] >> 8) & m1;
-      const float uu = u1[rho], ur = lane_from_right(u0[rho]), vv = v1[rho];
-      const float vt = (rho + 1 < R) ? v1[rho + 1 < R ? rho + 1 : rho] : vt1;
-      const float div = ((ur - uu) + vt) - vv;
-      const float d = relaxed_quotient(div, g, o);
-      if constexpr (PRESSURE) {
-        if (g & G_UPD) s1[rho] = s1[rho] + pressure_term(d, density, csf, d_t);
-      }
-      u1[rho] = gated_add(uu, d, g, 0);
-      u0[rho] = u0[rho] - lane_from_left(gate_pos(d, g, 1));
-      v1[rho] = gated_add(vv, d, g, 2);
-      if (rho + 1 < R)
-        v1[rho + 1 < R ? rho + 1 : rho] = gated_sub(vt, d, g, 3);
-      else
-        dtop = gate_pos(d, g, 3);
+// Register state of one lane of the register-resident kernel.
+template <int R>
+struct RegTile {
+  float u0[R], u1[R], v0[R], v1[R];  // faces of cells 2l and 2l+1, rows 0..R-1
+  float s0[R], s1[R];                // pressure sums (pressure runs only)
+  uint32_t gw[R];                    // gate bytes: cell 2l in bits 0-7, 2l+1 in 8-15
+};
+
+// apply_projection_at for the active cell of row rho (parity B) of every
+// lane.  mr masks rows outside the box, m0/m1 lanes outside it.
+template <int R, int B, bool PRESSURE>
+__device__ inline void reg_row(RegTile<R>& t, const int rho, uint32_t mr, uint32_t m0,
+                               uint32_t m1, float vt0, float vt1, float& dtop, float o,
+                               float density, float csf, float d_t) {
+  const int up = rho + 1 < R ? rho + 1 : rho;  // row above (when inside the strip)
+  if constexpr (B == 0) {
+    // active cell 2l: faces u0, u1, v0[rho], v0[rho+1] (all in this lane)
+    const uint32_t g = t.gw[rho] & m0 & mr;
+    const float uu = t.u0[rho], ur = t.u1[rho], vv = t.v0[rho];
+    const float vt = (rho + 1 < R) ? t.v0[up] : vt0;
+    const float div = ((ur - uu) + vt) - vv;
+    const float d = relaxed_quotient(div, g, o);
+    if constexpr (PRESSURE) {
+      if (g & G_UPD) t.s0[rho] = t.s0[rho] + pressure_term(d, density, csf, d_t);
     }
+    t.u0[rho] = gated_add(uu, d, g, 0);
+    t.u1[rho] = gated_sub(ur, d, g, 1);
+    t.v0[rho] = gated_add(vv, d, g, 2);
+    if (rho + 1 < R)
+      t.v0[up] = gated_sub(vt, d, g, 3);
+    else
+      dtop = gate_pos(d, g, 3);
+  } else {
+    // active cell 2l+1: faces u1, u0 of lane l+1, v1[rho], v1[rho+1]
+    const uint32_t g = (t.gw[rho] >> 8) & m1 & mr;
+    const float uu = t.u1[rho], ur = lane_from_right(t.u0[rho]), vv = t.v1[rho];
+    const float vt = (rho + 1 < R) ? t.v1[up] : vt1;
+    const float div = ((ur - uu) + vt) - vv;
+    const float d = relaxed_quotient(div, g, o);
+    if constexpr (PRESSURE) {
+      if (g & G_UPD) t.s1[rho] = t.s1[rho] + pressure_term(d, density, csf, d_t);
+    }
+    t.u1[rho] = gated_add(uu, d, g, 0);
+    t.u0[rho] = t.u0[rho] - lane_from_left(gate_pos(d, g, 1));
+    t.v1[rho] = gated_add(vv, d, g, 2);
+    if (rho + 1 < R)
+      t.v1[up] = gated_sub(vt, d, g, 3);
+    else
+      dtop = gate_pos(d, g, 3);
+  }
+}
+
+// Rows RHO..R-1 of one phase, unrolled at compile time.
+template <int R, int PB, bool PRESSURE, int RHO>
+__device__ inline void reg_rows_from(RegTile<R>& t, int rlo, int rhi_rel, uint32_t m0, uint32_t m1,
+                                     float vt0, float vt1, float& dtop, float o, float density,
+                                     float csf, float d_t) {
+  if constexpr (RHO < R) {
+    const uint32_t mr = (RHO >= rlo && RHO < rhi_rel) ? 0xffu : 0u;  // wavefront-uniform
+    reg_row<R, ((PB + RHO) & 1), PRESSURE>(t, RHO, mr, m0, m1, vt0, vt1, dtop, o, density, csf,
+                                           d_t);
+    reg_rows_from<R, PB, PRESSURE, RHO + 1>(t, rlo, rhi_rel, m0, m1, vt0, vt1, dtop, o, density,
+                                            csf, d_t);
+  }
+}
+
+// One colour phase.  PB is the parity of the active cell in row 0 of this
+// wavefront (compile time: phases come in pairs with PB and 1 - PB).  Rows
+// outside the box of cells that still reach the tile are masked rather
+// than skipped, so the phase is straight-line code over registers.
+template <int R, int PB, bool PRESSURE>
+__device__ inline void reg_phase(RegTile<R>& t, float2 (*vrow)[64], float (*dsend)[64],
+                                 int lane, int w, int rbase, int s, int nphase, int K, int tw,
+                                 int th, float o, float density, float csf, float d_t) {
+  const int e = nphase - 1 - s;
+  const int lo = 2 * K - 1 - e;
+  const int chi = 2 * K + tw + e;
+  const int rhi = 2 * K + th + e;
+  const int lc = 2 * lane;
+  const uint32_t m0 = (lc >= lo && lc < chi) ? 0xffu : 0u;
+  const uint32_t m1 = (lc + 1 >= lo && lc + 1 < chi) ? 0xffu : 0u;
+  const int rlo = lo - rbase, rhi_rel = rhi - rbase;  // box rows of this wavefront
+  // publish row 0 for the wavefront below, read the row above
+  if (w > 0) vrow[w][lane] = make_float2(t.v0[0], t.v1[0]);
+  __syncthreads();
+  float vt0 = 0.0f, vt1 = 0.0f;
+  if (w < kRegWaves - 1) {
+    const float2 x = vrow[w + 1][lane];
+    vt0 = x.x;
+    vt1 = x.y;
+  }
+  float dtop = 0.0f;
+  if (rlo < R && rhi_rel > 0)  // wavefront-uniform: some row is in the box
+    reg_rows_from<R, PB, PRESSURE, 0>(t, rlo, rhi_rel, m0, m1, vt0, vt1, dtop, o, density, csf,
+                                      d_t);
+  if (w < kRegWaves - 1) dsend[w][lane] = dtop;
+  __syncthreads();
+  if (w > 0) {
+    // the wavefront below moved the top faces of its last row: they are
+    // this wavefront's row-0 v of parity 1 - PB
+    const float dd = dsend[w - 1][lane];
+    if constexpr (PB == 0)
+      t.v1[0] = t.v1[0] - dd;
+    else
+      t.v0[0] = t.v0[0] - dd;
   }
 }
 
@@ -429,8 +489,7 @@ __global__ void __launch_bounds__(64 * kRegWaves)
   const int gi = gx0 + 2 * lane;
   const bool col0 = (unsigned)gi < (unsigned)W, col1 = (unsigned)(gi + 1) < (unsigned)W;
 
-  float u0[R], u1[R], v0[R], v1[R], s0[R], s1[R];
-  uint32_t gw[R];
+  RegTile<R> t;
   // flags: rows rbase .. rbase+R (one above for the top faces), columns
   // 2l, 2l+1 of this lane and 2l+2 (lane l+1's first column; lane 63 loads
   // its own)
@@ -451,57 +510,36 @@ __global__ void __launch_bounds__(64 * kRegWaves)
     const int gj = gy0 + rbase + rho;
     const bool row_in = (unsigned)gj < (unsigned)H;
     const long gk = (long)gj * P + gi;
-    load_pair(u_in, gk, row_in && col0, row_in && col1, u0[rho], u1[rho]);
-    load_pair(v_in, gk, row_in && col0, row_in && col1, v0[rho], v1[rho]);
+    load_pair(u_in, gk, row_in && col0, row_in && col1, t.u0[rho], t.u1[rho]);
+    load_pair(v_in, gk, row_in && col0, row_in && col1, t.v0[rho], t.v1[rho]);
     if constexpr (PRESSURE) {
-      s0[rho] = (row_in && col0) ? p[gk] : 0.0f;
-      s1[rho] = (row_in && col1) ? p[gk + 1] : 0.0f;
+      t.s0[rho] = (row_in && col0) ? p[gk] : 0.0f;
+      t.s1[rho] = (row_in && col1) ? p[gk + 1] : 0.0f;
     } else {
-      s0[rho] = s1[rho] = 0.0f;
+      t.s0[rho] = t.s1[rho] = 0.0f;
     }
-    gw[rho] = gate_byte(fl0[rho], fl1[rho], fl0[rho + 1]) |
-              (gate_byte(fl1[rho], fl2[rho], fl1[rho + 1]) << 8);
+    t.gw[rho] = gate_byte(fl0[rho], fl1[rho], fl0[rho + 1]) |
+                (gate_byte(fl1[rho], fl2[rho], fl1[rho + 1]) << 8);
   }
 
   const float o = sp.o;
   const float csf = (float)sp.cs;
   const int nphase = 2 * a.sweeps;
-  for (int s = 0; s < nphase; s++) {
-    const int c = s & 1;
-    const int e = nphase - 1 - s;
-    const int lo = 2 * K - 1 - e;
-    const int chi = 2 * K + a.tw + e;
-    const int rhi = 2 * K + a.th + e;
-    const int lc = 2 * lane;
-    const uint32_t m0 = (lc >= lo && lc < chi) ? 0xffu : 0u;
-    const uint32_t m1 = (lc + 1 >= lo && lc + 1 < chi) ? 0xffu : 0u;
-    // publish row 0 for the wavefront below, read the row above
-    if (w > 0) vrow[w][lane] = make_float2(v0[0], v1[0]);
-    __syncthreads();
-    float vt0 = 0.0f, vt1 = 0.0f;
-    if (w < kRegWaves - 1) {
-      const float2 t = vrow[w + 1][lane];
-      vt0 = t.x;
-      vt1 = t.y;
+  // colour 0 (even cells) first; row 0's active parity in a colour-0 phase
+  // is (gy0 + rbase) & 1 and flips every phase
+  if (((gy0 + rbase) & 1) == 0) {
+    for (int s = 0; s < nphase; s += 2) {
+      reg_phase<R, 0, PRESSURE>(t, vrow, dsend, lane, w, rbase, s, nphase, K, a.tw, a.th, o,
+                                sp.density, csf, sp.d_t);
+      reg_phase<R, 1, PRESSURE>(t, vrow, dsend, lane, w, rbase, s + 1, nphase, K, a.tw, a.th, o,
+                                sp.density, csf, sp.d_t);
     }
-    const int pb = (c + gy0 + rbase) & 1;  // parity of row 0's active cell
-    float dtop = 0.0f;
-    if (pb == 0)
-      reg_phase_rows<R, 0, PRESSURE>(u0, u1, v0, v1, gw, s0, s1, rbase, lo, rhi, m0, m1, vt0,
-                                     vt1, dtop, o, sp.density, csf, sp.d_t);
-    else
-      reg_phase_rows<R, 1, PRESSURE>(u0, u1, v0, v1, gw, s0, s1, rbase, lo, rhi, m0, m1, vt0,
-                                     vt1, dtop, o, sp.density, csf, sp.d_t);
-    if (w < kRegWaves - 1) dsend[w][lane] = dtop;
-    __syncthreads();
-    if (w > 0) {
-      // the wavefront below updated the top faces of its last row: those are
-      // this wavefront's row-0 v of the parity opposite to pb
-      const float dd = dsend[w - 1][lane];
-      if (pb == 0)
-        v1[0] = v1[0] - dd;
-      else
-        v0[0] = v0[0] - dd;
+  } else {
+    for (int s = 0; s < nphase; s += 2) {
+      reg_phase<R, 1, PRESSURE>(t, vrow, dsend, lane, w, rbase, s, nphase, K, a.tw, a.th, o,
+                                sp.density, csf, sp.d_t);
+      reg_phase<R, 0, PRESSURE>(t, vrow, dsend, lane, w, rbase, s + 1, nphase, K, a.tw, a.th, o,
+                                sp.density, csf, sp.d_t);
     }
   }
 
@@ -514,11 +552,11 @@ __global__ void __launch_bounds__(64 * kRegWaves)
     const int gj = gy0 + r;
     if (r < 2 * K || r >= 2 * K + a.th || gj >= H) continue;
     const long gk = (long)gj * P + gi;
-    store_pair(u_out, gk, own0, own1, u0[rho], u1[rho]);
-    store_pair(v_out, gk, own0, own1, v0[rho], v1[rho]);
+    store_pair(u_out, gk, own0, own1, t.u0[rho], t.u1[rho]);
+    store_pair(v_out, gk, own0, own1, t.v0[rho], t.v1[rho]);
     if constexpr (PRESSURE) {
-      if (own0) p[gk] = s0[rho];
-      if (own1) p[gk + 1] = s1[rho];
+      if (own0) p[gk] = t.s0[rho];
+      if (own1) p[gk + 1] = t.s1[rho];
     }
   }
 }

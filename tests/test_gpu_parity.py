@@ -201,7 +201,7 @@ TUNINGS = [
     dict(projection_kernel=1, tile_width=480, tile_height=8, sweeps_per_launch=2),
     dict(projection_kernel=2, tile_width=120, tile_height=24, sweeps_per_launch=2),
     dict(projection_kernel=2, tile_width=112, tile_height=48, sweeps_per_launch=4),
-    dict(projection_kernel=2, tile_width=116, tile_height=116, sweeps_per_launch=3),
+    dict(projection_kernel=2, tile_width=116, tile_height=52, sweeps_per_launch=3),
     dict(projection_kernel=2, tile_width=124, tile_height=28, sweeps_per_launch=1),
 ]
 
