@@ -9,6 +9,16 @@
 // the FP32 path bit-identical to a sequential IEEE evaluation of the
 // reference's update rule.
 //
+// With cell_size 1 (template flag CS1) a few steps are written in a cheaper
+// form that gives the same bits: x / 1 == x for every float x; the double
+// comparisons in_y <= 1/2.0 and in_x < 1/2.0 equal the float comparisons
+// with 0.5f; and (float)(0.5 - (double)t) == 0.5f - t for every float t in
+// [0, 1) (the double difference is exact for t >= 2^-29, and for smaller t
+// both sides round to 0.5f), likewise (float)((double)t - 0.5) == t - 0.5f
+// for t in [0.5, 1) (exact in both).  The sample position has 0 <= t < 1
+// whenever its cell is in the grid; outside the grid the result is 0 either
+// way.
+//
 // Device layout (ours, not the reference's): a field is H rows of `pitch`
 // elements, row j holding cells (0..W-1, j), j = 0 the bottom row.  The
 // boundary converts to the reference's indx order on copy.
@@ -58,11 +68,11 @@ struct GridView {
   int W, H, P, cs;
   const uint8_t* flags;
   __device__ long at(int i, int j) const { return (long)j * P + i; }
-  // Fluid::is_valid_fluid (src/fluid.cu:356-362)
-  __device__ bool valid(int i, int j) const {
-    return (unsigned)i < (unsigned)W && (unsigned)j < (unsigned)H &&
-           (flags[at(i, j)] & CF_FLUID);
+  __device__ bool in(int i, int j) const {
+    return (unsigned)i < (unsigned)W && (unsigned)j < (unsigned)H;
   }
+  // Fluid::is_valid_fluid (src/fluid.cu:356-362)
+  __device__ bool valid(int i, int j) const { return in(i, j) && (flags[at(i, j)] & CF_FLUID); }
 };
 
 // int i = x / cell_size (src/fluid.cu:419-420, 480-481, 645-646).  The
@@ -75,106 +85,121 @@ __device__ inline int to_index(float q) {
   return (int)q;
 }
 
-// get_general_velocity_x (src/fluid.cu:479-539).
-template <typename T>
-__device__ inline float sample_u(const GridView& g, const T* u, float x, float y) {
-  const int cs = g.cs;
-  const int i = to_index(x / (float)cs);
-  const int j = to_index(y / (float)cs);
-  if (!g.valid(i, j)) return 0.0f;
-  const float in_x = x - (float)(i * cs);
-  const float in_y = y - (float)(j * cs);
+// x / cs, with cs == 1 known at compile time when CS1.
+template <bool CS1> __device__ inline float div_cs(float x, int cs) {
+  if constexpr (CS1) return x;
+  else return x / (float)cs;
+}
+
+// Quotient x / n for a neighbour count n in 1..4, as IEEE division: n = 1,
+// 2, 4 multiply by the exact reciprocal, n = 3 divides.
+__device__ inline float div_count(float x, int n) {
+  if (n == 3) return x / 3.0f;
+  return x * __uint_as_float(0x3F800000u - ((uint32_t)(n >> 1) << 23));
+}
+
+// get_general_velocity_x (src/fluid.cu:479-539).  valid(i, j) is
+// Fluid::is_valid_fluid, val(i, j) reads u.
+template <bool CS1, class Valid, class Val>
+__device__ inline float sample_u(int cs, float x, float y, const Valid& valid, const Val& val) {
+  const int i = to_index(div_cs<CS1>(x, cs));
+  const int j = to_index(div_cs<CS1>(y, cs));
+  if (!valid(i, j)) return 0.0f;
+  const float in_x = x - (float)(CS1 ? i : i * cs);
+  const float in_y = y - (float)(CS1 ? j : j * cs);
   const double h = cs / 2.0;
   float acc = 0.0f;
-  if ((double)in_y <= h) {
-    const float d_y = (float)(h - (double)in_y);
-    const float w_x = 1.0f - in_x / (float)cs;
-    const float w_y = 1.0f - d_y / (float)cs;
-    acc = acc + (w_y * w_x) * ldf(u, g.at(i, j));
-    if (g.valid(i + 1, j)) acc = acc + (w_y * (1.0f - w_x)) * ldf(u, g.at(i + 1, j));
-    if (g.valid(i, j - 1)) acc = acc + ((1.0f - w_y) * w_x) * ldf(u, g.at(i, j - 1));
-    if (g.valid(i + 1, j - 1))
-      acc = acc + ((1.0f - w_y) * (1.0f - w_x)) * ldf(u, g.at(i + 1, j - 1));
+  const bool low = CS1 ? (in_y <= 0.5f) : ((double)in_y <= h);
+  if (low) {
+    const float d_y = CS1 ? 0.5f - in_y : (float)(h - (double)in_y);
+    const float w_x = 1.0f - div_cs<CS1>(in_x, cs);
+    const float w_y = 1.0f - div_cs<CS1>(d_y, cs);
+    acc = acc + (w_y * w_x) * val(i, j);
+    if (valid(i + 1, j)) acc = acc + (w_y * (1.0f - w_x)) * val(i + 1, j);
+    if (valid(i, j - 1)) acc = acc + ((1.0f - w_y) * w_x) * val(i, j - 1);
+    if (valid(i + 1, j - 1)) acc = acc + ((1.0f - w_y) * (1.0f - w_x)) * val(i + 1, j - 1);
   } else {
-    const float d_y = (float)((double)in_y - h);
-    const float w_x = 1.0f - in_x / (float)cs;
-    const float w_y = 1.0f - d_y / (float)cs;
-    acc = acc + (w_y * w_x) * ldf(u, g.at(i, j));
-    if (g.valid(i, j + 1)) acc = acc + ((1.0f - w_y) * w_x) * ldf(u, g.at(i, j + 1));
-    if (g.valid(i + 1, j)) acc = acc + (w_y * (1.0f - w_x)) * ldf(u, g.at(i + 1, j));
-    if (g.valid(i + 1, j + 1))
-      acc = acc + ((1.0f - w_y) * (1.0f - w_x)) * ldf(u, g.at(i + 1, j + 1));
+    const float d_y = CS1 ? in_y - 0.5f : (float)((double)in_y - h);
+    const float w_x = 1.0f - div_cs<CS1>(in_x, cs);
+    const float w_y = 1.0f - div_cs<CS1>(d_y, cs);
+    acc = acc + (w_y * w_x) * val(i, j);
+    if (valid(i, j + 1)) acc = acc + ((1.0f - w_y) * w_x) * val(i, j + 1);
+    if (valid(i + 1, j)) acc = acc + (w_y * (1.0f - w_x)) * val(i + 1, j);
+    if (valid(i + 1, j + 1)) acc = acc + ((1.0f - w_y) * (1.0f - w_x)) * val(i + 1, j + 1);
   }
   return acc;
 }
 
 // get_general_velocity_y (src/fluid.cu:418-477); strict `<` on x.
-template <typename T>
-__device__ inline float sample_v(const GridView& g, const T* v, float x, float y) {
-  const int cs = g.cs;
-  const int i = to_index(x / (float)cs);
-  const int j = to_index(y / (float)cs);
-  if (!g.valid(i, j)) return 0.0f;
-  const float in_x = x - (float)(i * cs);
-  const float in_y = y - (float)(j * cs);
+template <bool CS1, class Valid, class Val>
+__device__ inline float sample_v(int cs, float x, float y, const Valid& valid, const Val& val) {
+  const int i = to_index(div_cs<CS1>(x, cs));
+  const int j = to_index(div_cs<CS1>(y, cs));
+  if (!valid(i, j)) return 0.0f;
+  const float in_x = x - (float)(CS1 ? i : i * cs);
+  const float in_y = y - (float)(CS1 ? j : j * cs);
   const double h = cs / 2.0;
   float acc = 0.0f;
-  if ((double)in_x < h) {
-    const float d_x = (float)(h - (double)in_x);
-    const float w_x = 1.0f - d_x / (float)cs;
-    const float w_y = 1.0f - in_y / (float)cs;
-    acc = acc + (w_y * w_x) * ldf(v, g.at(i, j));
-    if (g.valid(i - 1, j)) acc = acc + (w_y * (1.0f - w_x)) * ldf(v, g.at(i - 1, j));
-    if (g.valid(i - 1, j + 1))
-      acc = acc + ((1.0f - w_y) * (1.0f - w_x)) * ldf(v, g.at(i - 1, j + 1));
-    if (g.valid(i, j + 1)) acc = acc + ((1.0f - w_y) * w_x) * ldf(v, g.at(i, j + 1));
+  const bool left = CS1 ? (in_x < 0.5f) : ((double)in_x < h);
+  if (left) {
+    const float d_x = CS1 ? 0.5f - in_x : (float)(h - (double)in_x);
+    const float w_x = 1.0f - div_cs<CS1>(d_x, cs);
+    const float w_y = 1.0f - div_cs<CS1>(in_y, cs);
+    acc = acc + (w_y * w_x) * val(i, j);
+    if (valid(i - 1, j)) acc = acc + (w_y * (1.0f - w_x)) * val(i - 1, j);
+    if (valid(i - 1, j + 1)) acc = acc + ((1.0f - w_y) * (1.0f - w_x)) * val(i - 1, j + 1);
+    if (valid(i, j + 1)) acc = acc + ((1.0f - w_y) * w_x) * val(i, j + 1);
   } else {
-    const float d_x = (float)((double)in_x - h);
-    const float w_x = 1.0f - d_x / (float)cs;
-    const float w_y = 1.0f - in_y / (float)cs;
-    acc = acc + (w_y * w_x) * ldf(v, g.at(i, j));
-    if (g.valid(i, j + 1)) acc = acc + ((1.0f - w_y) * w_x) * ldf(v, g.at(i, j + 1));
-    if (g.valid(i + 1, j + 1))
-      acc = acc + ((1.0f - w_y) * (1.0f - w_x)) * ldf(v, g.at(i + 1, j + 1));
-    if (g.valid(i + 1, j)) acc = acc + (w_y * (1.0f - w_x)) * ldf(v, g.at(i + 1, j));
+    const float d_x = CS1 ? in_x - 0.5f : (float)((double)in_x - h);
+    const float w_x = 1.0f - div_cs<CS1>(d_x, cs);
+    const float w_y = 1.0f - div_cs<CS1>(in_y, cs);
+    acc = acc + (w_y * w_x) * val(i, j);
+    if (valid(i, j + 1)) acc = acc + ((1.0f - w_y) * w_x) * val(i, j + 1);
+    if (valid(i + 1, j + 1)) acc = acc + ((1.0f - w_y) * (1.0f - w_x)) * val(i + 1, j + 1);
+    if (valid(i + 1, j)) acc = acc + (w_y * (1.0f - w_x)) * val(i + 1, j);
   }
   return acc;
 }
 
-// get_center_position (src/fluid.cu:547-550).
-__device__ inline float centre(int a, int cs) { return (float)(((double)a + 0.5) * (double)cs); }
+// get_center_position (src/fluid.cu:547-550): ((a + 0.5) * cs) in double,
+// rounded to float.  With cs == 1, (float)a + 0.5f rounds the same exact value.
+template <bool CS1> __device__ inline float centre(int a, int cs) {
+  if constexpr (CS1) return (float)a + 0.5f;
+  else return (float)(((double)a + 0.5) * (double)cs);
+}
 
-// interpolate_smoke (src/fluid.cu:644-716) with get_distance
-// (inc/helper.cuh:77-81).
-template <typename T>
-__device__ inline float sample_smoke(const GridView& g, const T* s, float x, float y) {
-  const int cs = g.cs;
-  const int i = to_index(x / (float)cs);
-  const int j = to_index(y / (float)cs);
-  const float in_x = x - (float)(i * cs);
-  const float in_y = y - (float)(j * cs);
+// 1.0 / (distance + 1e-6) in double, rounded to float (src/fluid.cu:690-693),
+// with get_distance (inc/helper.cuh:77-81).
+__device__ inline float inv_distance(float dx, float dy) {
+  return (float)(1.0 / ((double)sqrtf(dx * dx + dy * dy) + 1e-6));
+}
+
+// interpolate_smoke (src/fluid.cu:644-716).
+template <bool CS1, class Valid, class Val>
+__device__ inline float sample_smoke(int cs, float x, float y, const Valid& valid,
+                                     const Val& val) {
+  const int i = to_index(div_cs<CS1>(x, cs));
+  const int j = to_index(div_cs<CS1>(y, cs));
+  const float in_x = x - (float)(CS1 ? i : i * cs);
+  const float in_y = y - (float)(CS1 ? j : j * cs);
   const double h = cs / 2.0;
-  const bool left = (double)in_x < h, low = (double)in_y < h;
+  const bool left = CS1 ? (in_x < 0.5f) : ((double)in_x < h);
+  const bool low = CS1 ? (in_y < 0.5f) : ((double)in_y < h);
   const int i2 = left ? i - 1 : i + 1;
   const int j3 = low ? j - 1 : j + 1;
   // cells: 1 = (i, j), 2 = (i2, j), 3 = (i, j3), 4 = (i2, j3)
-  const float x1 = centre(i, cs), y1 = centre(j, cs);
-  const float x2 = centre(i2, cs), y3 = centre(j3, cs);
-  float dx, dy;
-  dx = x - x1; dy = y - y1;
-  const float inv1 = (float)(1.0 / ((double)sqrtf(dx * dx + dy * dy) + 1e-6));
-  dx = x - x2; dy = y - y1;
-  const float inv2 = (float)(1.0 / ((double)sqrtf(dx * dx + dy * dy) + 1e-6));
-  dx = x - x1; dy = y - y3;
-  const float inv3 = (float)(1.0 / ((double)sqrtf(dx * dx + dy * dy) + 1e-6));
-  dx = x - x2; dy = y - y3;
-  const float inv4 = (float)(1.0 / ((double)sqrtf(dx * dx + dy * dy) + 1e-6));
+  const float x1 = centre<CS1>(i, cs), y1 = centre<CS1>(j, cs);
+  const float x2 = centre<CS1>(i2, cs), y3 = centre<CS1>(j3, cs);
+  const float inv1 = inv_distance(x - x1, y - y1);
+  const float inv2 = inv_distance(x - x2, y - y1);
+  const float inv3 = inv_distance(x - x1, y - y3);
+  const float inv4 = inv_distance(x - x2, y - y3);
   const float sum_inv = ((inv1 + inv2) + inv3) + inv4;
   float acc = 0.0f;
-  if (g.valid(i, j)) acc = acc + (inv1 / sum_inv) * ldf(s, g.at(i, j));
-  if (g.valid(i2, j)) acc = acc + (inv2 / sum_inv) * ldf(s, g.at(i2, j));
-  if (g.valid(i, j3)) acc = acc + (inv3 / sum_inv) * ldf(s, g.at(i, j3));
-  if (g.valid(i2, j3)) acc = acc + (inv4 / sum_inv) * ldf(s, g.at(i2, j3));
+  if (valid(i, j)) acc = acc + (inv1 / sum_inv) * val(i, j);
+  if (valid(i2, j)) acc = acc + (inv2 / sum_inv) * val(i2, j);
+  if (valid(i, j3)) acc = acc + (inv3 / sum_inv) * val(i, j3);
+  if (valid(i2, j3)) acc = acc + (inv4 / sum_inv) * val(i2, j3);
   return acc;
 }
 

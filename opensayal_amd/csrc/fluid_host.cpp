@@ -146,16 +146,16 @@ namespace {
 
 size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
-// Starting launch shape before any autotuning: 4 sweeps per launch, tiles
-// of 112 x 16 cells (a 128 x 32 region: one wavefront per region row, three
-// workgroups per CU), never wider than the grid.
+// Starting launch shape before any autotuning: the register-resident
+// kernel with 8 rows per wavefront and 5 sweeps per launch (a 128 x 64
+// region owning 108 x 44 cells), the fastest measured at 1920 x 1080.
 ProjTuning default_tuning(int W, bool pressure) {
+  (void)W;
   ProjTuning t;
-  t.kernel = PROJ_TILED;
-  t.k = 4;
-  t.tw = W >= 112 ? 112 : (W + 1) / 2 * 2;
-  t.th = 16;
-  while (t.th > 4 && !t.valid(pressure)) t.th -= 4;
+  t.kernel = PROJ_REGS;
+  t.k = 5;
+  t.tw = 128 - 4 * t.k;
+  t.th = 8 * kRegWaves - 4 * t.k;
   if (!t.valid(pressure)) t.kernel = PROJ_COLOUR;
   return t;
 }

@@ -37,6 +37,9 @@ struct StepParams {
   int smoke_on;           // enable_smoke && wind_tunnel.smoke != 0
   int decay_on;
   float decay_rd;         // decay_rate * d_t
+  // set by the step: the first projection launch applies the velocity
+  // forces to what it loads and starts the pressure sums at 0
+  int forces_in_projection;
 };
 
 // Device buffers of one simulation; T is the storage type of u, v, smoke.

@@ -18,6 +18,7 @@
 //    shrinks by one cell per phase), so the redundant halo work falls off as
 //    the sweeps proceed.
 #include "projection.hpp"
+#include "step_device.hpp"
 
 #pragma clang fp contract(off)
 
@@ -102,6 +103,7 @@ struct TiledArgs {
   int wpr;        // wavefronts per region row
   int ntx;        // tiles per grid row
   int sweeps;     // sweeps this launch (<= k)
+  int forces;     // apply the velocity forces on load, pressure sums from 0
 };
 
 // Two horizontally adjacent elements (cells 2q, 2q+1 of a row).
@@ -201,17 +203,21 @@ __global__ void __launch_bounds__(kTileThreads)
     const bool in0 = row_in && (unsigned)gi < (unsigned)W;
     const bool in1 = row_in && (unsigned)(gi + 1) < (unsigned)W;
     const long gk = (long)gj * P + gi;
-    float a0, a1;
+    float a0, a1, b0, b1;
     const int l0 = r * rs + q, l1 = l0 + pwp;
     load_pair(u_in, gk, in0, in1, a0, a1);
+    load_pair(v_in, gk, in0, in1, b0, b1);
+    if (a.forces) {
+      if (in0) force_uv(sp, gi, gj, a0, b0);
+      if (in1) force_uv(sp, gi + 1, gj, a1, b1);
+    }
     U[l0] = a0;
     U[l1] = a1;
-    load_pair(v_in, gk, in0, in1, a0, a1);
-    V[l0] = a0;
-    V[l1] = a1;
+    V[l0] = b0;
+    V[l1] = b1;
     if constexpr (PRESSURE) {
-      S[l0] = in0 ? p[gk] : 0.0f;
-      S[l1] = in1 ? p[gk + 1] : 0.0f;
+      S[l0] = (in0 && !a.forces) ? p[gk] : 0.0f;
+      S[l1] = (in1 && !a.forces) ? p[gk + 1] : 0.0f;
     }
   }
   __syncthreads();
@@ -354,6 +360,7 @@ struct RegArgs {
   int tw, th, k;  // owned tile and sweeps per launch (halo 2k)
   int ntx;        // tiles per grid row
   int sweeps;     // sweeps this launch (<= k)
+  int forces;     // apply the velocity forces on load, pressure sums from 0
 };
 
 __device__ inline bool lc_in_tile(int lc, int K, int tw) { return lc >= 2 * K && lc < 2 * K + tw; }
@@ -512,9 +519,13 @@ __global__ void __launch_bounds__(64 * kRegWaves)
     const long gk = (long)gj * P + gi;
     load_pair(u_in, gk, row_in && col0, row_in && col1, t.u0[rho], t.u1[rho]);
     load_pair(v_in, gk, row_in && col0, row_in && col1, t.v0[rho], t.v1[rho]);
+    if (a.forces) {
+      if (row_in && col0) force_uv(sp, gi, gj, t.u0[rho], t.v0[rho]);
+      if (row_in && col1) force_uv(sp, gi + 1, gj, t.u1[rho], t.v1[rho]);
+    }
     if constexpr (PRESSURE) {
-      t.s0[rho] = (row_in && col0) ? p[gk] : 0.0f;
-      t.s1[rho] = (row_in && col1) ? p[gk + 1] : 0.0f;
+      t.s0[rho] = (row_in && col0 && !a.forces) ? p[gk] : 0.0f;
+      t.s1[rho] = (row_in && col1 && !a.forces) ? p[gk + 1] : 0.0f;
     } else {
       t.s0[rho] = t.s1[rho] = 0.0f;
     }
@@ -576,6 +587,7 @@ void launch_tiled(const StepParams& sp, const GridView& g, const FieldPtrs& f, i
   const size_t lds = t.lds_bytes(PRESSURE);
   for (int done = 0; done < sp.n; done += t.k) {
     a.sweeps = (sp.n - done < t.k) ? sp.n - done : t.k;
+    a.forces = (done == 0 && sp.forces_in_projection) ? 1 : 0;
     const int nxt = 1 - *cur;
     hipLaunchKernelGGL((k_project_tiled<T, PRESSURE>), dim3(a.ntx * nty), dim3(kTileThreads), lds,
                        s, sp, g.flags, (const T*)f.u[*cur], (const T*)f.v[*cur], (T*)f.u[nxt],
@@ -595,6 +607,7 @@ void launch_regs_r(const StepParams& sp, const GridView& g, const FieldPtrs& f, 
   const int nty = (sp.H + a.th - 1) / a.th;
   for (int done = 0; done < sp.n; done += t.k) {
     a.sweeps = (sp.n - done < t.k) ? sp.n - done : t.k;
+    a.forces = (done == 0 && sp.forces_in_projection) ? 1 : 0;
     const int nxt = 1 - *cur;
     hipLaunchKernelGGL((k_project_regs<T, PRESSURE, R>), dim3(a.ntx * nty), dim3(64 * kRegWaves),
                        0, s, sp, g.flags, (const T*)f.u[*cur], (const T*)f.v[*cur], (T*)f.u[nxt],

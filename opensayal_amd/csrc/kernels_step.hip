@@ -1,14 +1,20 @@
 // kernels_step.hip - the per-step phases of Fluid::update (src/fluid.cu:770-795)
 // as HIP kernels for gfx950, plus mask build, reductions and layout copies.
 //
-// Phase kernels read and write HBM once per phase.  u/v and smoke advection
-// write into the other half of a ping-pong pair instead of the reference's
-// buffer-then-copy-back (src/fluid.cu:614-617, 569-571), which gives the same
-// values with half the traffic.  All kernels launch 64-wide rows of lanes
-// along i (one wavefront = 64 consecutive cells of a row, coalesced).
+// One step is: the projection launches (the first of which applies the
+// external forces to every cell it loads), the velocity advection (which
+// reads the projected field through the edge extrapolation, so the
+// extrapolated field is never stored), and the smoke advection (which
+// applies the smoke part of the forces as it loads and the decay as it
+// stores).  Both advections write into the other half of a ping-pong pair
+// instead of the reference's buffer-then-copy-back (src/fluid.cu:614-617,
+// 569-571): same values, half the traffic.  The advections stage a tile of
+// the fields plus a 2-cell halo in LDS with coalesced loads; a back-trace
+// that leaves the staged region reads HBM directly.
 #include "fluid_device.hpp"
 #include "kernels.hpp"
 #include "projection.hpp"
+#include "step_device.hpp"
 
 #pragma clang fp contract(off)
 
@@ -64,47 +70,27 @@ __global__ void k_build_flags(MaskParams m, uint8_t* flags) {
 }
 
 // ---- P1: apply_external_forces_at (src/fluid.cu:308-349), zero_pressure ---
-
+// Only the parts no later kernel applies while loading: the velocity part
+// when no tiled projection launch follows, the smoke part when no smoke
+// advection follows, and the pressure zeroing in the first case.
 template <typename T>
-__global__ void k_forces(StepParams sp, T* u, T* v, T* smoke, float* p) {
+__global__ void k_forces(StepParams sp, int do_uv, int do_smoke, T* u, T* v, T* smoke, float* p) {
   const int i = blockIdx.x * kBX + threadIdx.x;
   const int j = blockIdx.y * kBY + threadIdx.y;
   if (i >= sp.W || j >= sp.H) return;
   const long k = (long)j * sp.P + i;
-  float uu = ldf(u, k), vv = ldf(v, k);
-  bool u_set = false, sm_set = false;
-  float sm = 0.0f;
-  if (i <= sp.smoke_length && i != 0 && j >= sp.band_lo && j <= sp.band_hi) {
-    uu = sp.speed;
-    u_set = true;
-    if ((sp.smoke_count == 1 && j >= sp.sband_lo && j <= sp.sband_hi) ||
-        (sp.smoke_count != 1 && (sp.smoke_top - j) % sp.smoke_mod < sp.smoke_height)) {
-      sm = sp.wind_smoke;
-      sm_set = true;
-    }
+  if (do_uv) {
+    float uu = ldf(u, k), vv = ldf(v, k);
+    force_uv(sp, i, j, uu, vv);
+    stf(u, k, uu);
+    stf(v, k, vv);
+    if (sp.pressure_on) p[k] = 0.0f;
   }
-  if (sp.drag_on) {
-    uu = uu * sp.damping;
-    vv = vv * sp.damping;
-    u_set = true;
+  if (do_smoke) {
+    const float s0 = ldf(smoke, k);
+    const float s1 = force_smoke(sp, i, j, s0);
+    if (__float_as_uint(s1) != __float_as_uint(s0)) stf(smoke, k, s1);
   }
-  if (sp.src_active) {
-    const int dx = i - sp.src_x, dy = j - sp.src_y;
-    if (dx * dx + dy * dy < 40 * 40) {
-      if (sp.src_smoke != 0.0f) {
-        sm = sp.src_smoke;
-        sm_set = true;
-      }
-      uu = uu + sp.src_velocity * (float)dx;
-      vv = vv + sp.src_velocity * (float)dy;
-      u_set = true;
-    }
-  }
-  vv = vv + sp.gdt;
-  stf(v, k, vv);
-  if (u_set) stf(u, k, uu);
-  if (sm_set) stf(smoke, k, sm);
-  if (sp.pressure_on) p[k] = 0.0f;
 }
 
 // ---- P5: thrust::reduce min/max of pressure (src/fluid.cu:778-787) -------
@@ -168,90 +154,141 @@ __global__ void __launch_bounds__(256) k_minmax_final(const float* part, int npa
   }
 }
 
-// ---- P6: apply_extrapolation (src/fluid.cu:720-747) ------------------------
-// Threads 0..W-1 do the row phase for column i, threads W..W+H-1 the column
-// phase for row j.  Reads and writes of the two phases are disjoint once the
-// row phase skips column 1 (whose row-0 and row-(H-1) u values the column
-// phase sets to 0) and the column phase uses 0 for v(1,1) and v(W-2,1) (which
-// the row phase zeroes), so one launch gives the two-phase result.
-template <typename T>
-__global__ void k_extrapolate(StepParams sp, T* u, T* v) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  const int W = sp.W, H = sp.H, P = sp.P;
-  if (t < W) {
-    const int i = t;
-    if (i != 1) {
-      stf(u, i, ldf(u, (long)P + i));                                // u(i,0) = u(i,1)
-      stf(u, (long)(H - 1) * P + i, ldf(u, (long)(H - 2) * P + i));  // u(i,H-1) = u(i,H-2)
-    }
-    stf(v, (long)P + i, 0.0f);  // v(i,1) = 0
-  } else if (t < W + H) {
-    const int j = t - W;
-    const long r = (long)j * P;
-    stf(v, r, j == 1 ? 0.0f : ldf(v, r + 1));                   // v(0,j) = v(1,j)
-    stf(u, r + 1, 0.0f);                                        // u(1,j) = 0
-    stf(v, r + W - 1, j == 1 ? 0.0f : ldf(v, r + W - 2));       // v(W-1,j) = v(W-2,j)
-  }
-}
+// ---- staged tiles for the advections ----------------------------------------
+// A workgroup of 64 x 4 lanes advects a 64 x 16 tile (4 rows per lane).  It
+// first stages the tile plus a 2-cell halo of every field it samples in LDS;
+// samples outside the staged region fall back to HBM reads of the same
+// values.
+constexpr int kAdvTX = 64, kAdvTY = 16, kAdvHalo = 2;
+constexpr int kAdvRX = kAdvTX + 2 * kAdvHalo, kAdvRY = kAdvTY + 2 * kAdvHalo;
+constexpr int kAdvThreads = 256;
 
-// ---- P7: apply_velocity_advection_at (src/fluid.cu:598-612) ---------------
-template <typename T>
-__global__ void k_advect_velocity(StepParams sp, GridView g, const T* u, const T* v, T* u_out,
-                                  T* v_out) {
-  const int i = blockIdx.x * kBX + threadIdx.x;
-  const int j = blockIdx.y * kBY + threadIdx.y;
-  if (i >= sp.W || j >= sp.H) return;
-  const long k = g.at(i, j);
+struct Region {
+  int x0, y0;  // grid coordinates of staged cell (0, 0)
+  __device__ bool in(int i, int j) const {
+    return (unsigned)(i - x0) < (unsigned)kAdvRX && (unsigned)(j - y0) < (unsigned)kAdvRY;
+  }
+  __device__ int at(int i, int j) const { return (j - y0) * kAdvRX + (i - x0); }
+};
+
+// ---- P6 + P7: apply_extrapolation then apply_velocity_advection_at
+// (src/fluid.cu:720-747, 598-612).  The staged u, v are the extrapolated
+// values (u_extrapolated / v_extrapolated), which are also what the HBM
+// fallback reads.
+template <typename T, bool CS1>
+__global__ void __launch_bounds__(kAdvThreads)
+    k_advect_velocity(StepParams sp, GridView g, const T* __restrict__ u,
+                      const T* __restrict__ v, T* __restrict__ u_out, T* __restrict__ v_out) {
+  __shared__ float su[kAdvRY * kAdvRX], sv[kAdvRY * kAdvRX];
+  __shared__ uint8_t sf[kAdvRY * kAdvRX];
+  const Region R{(int)blockIdx.x * kAdvTX - kAdvHalo, (int)blockIdx.y * kAdvTY - kAdvHalo};
+  const int tid = threadIdx.y * kAdvTX + threadIdx.x;
+  for (int q = tid; q < kAdvRX * kAdvRY; q += kAdvThreads) {
+    const int rr = q / kAdvRX, cc = q - rr * kAdvRX;
+    const int i = R.x0 + cc, j = R.y0 + rr;
+    const bool in = g.in(i, j);
+    su[q] = in ? u_extrapolated(u, g, i, j) : 0.0f;
+    sv[q] = in ? v_extrapolated(v, g, i, j) : 0.0f;
+    sf[q] = in ? g.flags[g.at(i, j)] : (uint8_t)0;
+  }
+  __syncthreads();
+  auto valid = [&](int i, int j) -> bool {
+    return R.in(i, j) ? (sf[R.at(i, j)] & CF_FLUID) != 0 : g.valid(i, j);
+  };
+  auto uval = [&](int i, int j) -> float {
+    return R.in(i, j) ? su[R.at(i, j)] : u_extrapolated(u, g, i, j);
+  };
+  auto vval = [&](int i, int j) -> float {
+    return R.in(i, j) ? sv[R.at(i, j)] : v_extrapolated(v, g, i, j);
+  };
   const int cs = sp.cs;
   const float dt = sp.d_t;
-  {  // u face at (i*cs, (j+0.5)*cs); get_vertical_edge_velocity (:364-389)
-    const float px = (float)(i * cs);
-    const float py = (float)(((double)j + 0.5) * (double)cs);
-    const float vel_x = ldf(u, k);
-    float avg_v = ldf(v, k);
-    int count = 1;
-    if (g.valid(i - 1, j + 1)) { avg_v = avg_v + ldf(v, g.at(i - 1, j + 1)); count++; }
-    if (g.valid(i, j + 1)) { avg_v = avg_v + ldf(v, g.at(i, j + 1)); count++; }
-    if (g.valid(i - 1, j)) { avg_v = avg_v + ldf(v, g.at(i - 1, j)); count++; }
-    avg_v = avg_v / (float)count;
-    const float x = px - vel_x * dt;
-    const float y = py - avg_v * dt;
-    stf(u_out, k, sample_u(g, u, x, y));
-  }
-  {  // v face at ((i+0.5)*cs, j*cs); get_horizontal_edge_velocity (:391-416)
-    const float px = (float)(((double)i + 0.5) * (double)cs);
-    const float py = (float)(j * cs);
-    const float vel_y = ldf(v, k);
-    float avg_u = ldf(u, k);
-    int count = 1;
-    if (g.valid(i + 1, j)) { avg_u = avg_u + ldf(u, g.at(i + 1, j)); count++; }
-    if (g.valid(i, j - 1)) { avg_u = avg_u + ldf(u, g.at(i, j - 1)); count++; }
-    if (g.valid(i + 1, j - 1)) { avg_u = avg_u + ldf(u, g.at(i + 1, j - 1)); count++; }
-    avg_u = avg_u / (float)count;
-    const float x = px - avg_u * dt;
-    const float y = py - vel_y * dt;
-    stf(v_out, k, sample_v(g, v, x, y));
+  const int i = blockIdx.x * kAdvTX + threadIdx.x;
+#pragma unroll
+  for (int m = 0; m < kAdvTY / 4; m++) {
+    const int j = blockIdx.y * kAdvTY + threadIdx.y + 4 * m;
+    if (i >= sp.W || j >= sp.H) continue;
+    const int l = R.at(i, j);
+    float nu, nv;
+    {  // u face at (i*cs, (j+0.5)*cs); get_vertical_edge_velocity (:364-389)
+      const float px = (float)(CS1 ? i : i * cs);
+      const float py = CS1 ? (float)j + 0.5f : (float)(((double)j + 0.5) * (double)cs);
+      const float vel_x = su[l];
+      float avg_v = sv[l];
+      int count = 1;
+      if (valid(i - 1, j + 1)) { avg_v = avg_v + vval(i - 1, j + 1); count++; }
+      if (valid(i, j + 1)) { avg_v = avg_v + vval(i, j + 1); count++; }
+      if (valid(i - 1, j)) { avg_v = avg_v + vval(i - 1, j); count++; }
+      avg_v = div_count(avg_v, count);
+      const float x = px - vel_x * dt;
+      const float y = py - avg_v * dt;
+      nu = sample_u<CS1>(cs, x, y, valid, uval);
+    }
+    {  // v face at ((i+0.5)*cs, j*cs); get_horizontal_edge_velocity (:391-416)
+      const float px = CS1 ? (float)i + 0.5f : (float)(((double)i + 0.5) * (double)cs);
+      const float py = (float)(CS1 ? j : j * cs);
+      const float vel_y = sv[l];
+      float avg_u = su[l];
+      int count = 1;
+      if (valid(i + 1, j)) { avg_u = avg_u + uval(i + 1, j); count++; }
+      if (valid(i, j - 1)) { avg_u = avg_u + uval(i, j - 1); count++; }
+      if (valid(i + 1, j - 1)) { avg_u = avg_u + uval(i + 1, j - 1); count++; }
+      avg_u = div_count(avg_u, count);
+      const float x = px - avg_u * dt;
+      const float y = py - vel_y * dt;
+      nv = sample_v<CS1>(cs, x, y, valid, vval);
+    }
+    stf(u_out, g.at(i, j), nu);
+    stf(v_out, g.at(i, j), nv);
   }
 }
 
 // ---- P8: apply_smoke_advection_at + decay_smoke_at (src/fluid.cu:560-567,
-// 758-762), fused: the decay is applied to the advected value before it is
-// stored, which is the reference's copy-back-then-decay for FP32 storage.
-template <typename T>
-__global__ void k_advect_smoke(StepParams sp, GridView g, const T* u, const T* v, const T* smoke,
-                               T* smoke_out) {
-  const int i = blockIdx.x * kBX + threadIdx.x;
-  const int j = blockIdx.y * kBY + threadIdx.y;
-  if (i >= sp.W || j >= sp.H) return;
+// 758-762).  The staged smoke is the smoke after the forces' inflow band and
+// source (force_smoke); the decay is applied to the advected value before it
+// is stored, which is the reference's copy-back-then-decay.
+template <typename T, bool CS1>
+__global__ void __launch_bounds__(kAdvThreads)
+    k_advect_smoke(StepParams sp, GridView g, const T* __restrict__ u, const T* __restrict__ v,
+                   const T* __restrict__ smoke, T* __restrict__ smoke_out) {
+  __shared__ float su[kAdvRY * kAdvRX], sv[kAdvRY * kAdvRX], ss[kAdvRY * kAdvRX];
+  __shared__ uint8_t sf[kAdvRY * kAdvRX];
+  const Region R{(int)blockIdx.x * kAdvTX - kAdvHalo, (int)blockIdx.y * kAdvTY - kAdvHalo};
+  const int tid = threadIdx.y * kAdvTX + threadIdx.x;
+  for (int q = tid; q < kAdvRX * kAdvRY; q += kAdvThreads) {
+    const int rr = q / kAdvRX, cc = q - rr * kAdvRX;
+    const int i = R.x0 + cc, j = R.y0 + rr;
+    const bool in = g.in(i, j);
+    const long k = in ? g.at(i, j) : 0;
+    su[q] = in ? ldf(u, k) : 0.0f;
+    sv[q] = in ? ldf(v, k) : 0.0f;
+    ss[q] = in ? force_smoke(sp, i, j, ldf(smoke, k)) : 0.0f;
+    sf[q] = in ? g.flags[k] : (uint8_t)0;
+  }
+  __syncthreads();
+  auto valid = [&](int i, int j) -> bool {
+    return R.in(i, j) ? (sf[R.at(i, j)] & CF_FLUID) != 0 : g.valid(i, j);
+  };
+  auto uval = [&](int i, int j) -> float { return R.in(i, j) ? su[R.at(i, j)] : ldf(u, g.at(i, j)); };
+  auto vval = [&](int i, int j) -> float { return R.in(i, j) ? sv[R.at(i, j)] : ldf(v, g.at(i, j)); };
+  auto sval = [&](int i, int j) -> float {
+    return R.in(i, j) ? ss[R.at(i, j)] : force_smoke(sp, i, j, ldf(smoke, g.at(i, j)));
+  };
   const int cs = sp.cs;
-  const float cx = centre(i, cs), cy = centre(j, cs);
-  const float vx = sample_u(g, u, cx, cy);
-  const float vy = sample_v(g, v, cx, cy);
-  const float x = cx - vx * sp.d_t;
-  const float y = cy - vy * sp.d_t;
-  float s = sample_smoke(g, smoke, x, y);
-  if (sp.decay_on) s = (float)fmax((double)(s - sp.decay_rd), 0.0);
-  stf(smoke_out, g.at(i, j), s);
+  const int i = blockIdx.x * kAdvTX + threadIdx.x;
+#pragma unroll
+  for (int m = 0; m < kAdvTY / 4; m++) {
+    const int j = blockIdx.y * kAdvTY + threadIdx.y + 4 * m;
+    if (i >= sp.W || j >= sp.H) continue;
+    const float cx = centre<CS1>(i, cs), cy = centre<CS1>(j, cs);
+    const float vx = sample_u<CS1>(cs, cx, cy, valid, uval);
+    const float vy = sample_v<CS1>(cs, cx, cy, valid, vval);
+    const float x = cx - vx * sp.d_t;
+    const float y = cy - vy * sp.d_t;
+    float s = sample_smoke<CS1>(cs, x, y, valid, sval);
+    if (sp.decay_on) s = (float)fmax((double)(s - sp.decay_rd), 0.0);
+    stf(smoke_out, g.at(i, j), s);
+  }
 }
 
 // ---- divergence norms (Fluid::get_divergence, src/fluid.cu:192-201) -------
@@ -331,35 +368,49 @@ __global__ void k_layout(int mode, int W, int H, int P, T* dev, float* pdev, con
   }
 }
 
+template <typename T, bool CS1>
+void advect_t(const StepParams& sp, const GridView& g, const FieldPtrs& f, int cur, int* uv,
+              int* sm, hipStream_t s) {
+  const dim3 grid((sp.W + kAdvTX - 1) / kAdvTX, (sp.H + kAdvTY - 1) / kAdvTY), block(kAdvTX, 4);
+  const int nxt = 1 - cur;
+  T* u2 = (T*)f.u[nxt];
+  T* v2 = (T*)f.v[nxt];
+  hipLaunchKernelGGL((k_advect_velocity<T, CS1>), grid, block, 0, s, sp, g, (const T*)f.u[cur],
+                     (const T*)f.v[cur], u2, v2);
+  *uv = nxt;
+  if (sp.smoke_on) {
+    hipLaunchKernelGGL((k_advect_smoke<T, CS1>), grid, block, 0, s, sp, g, (const T*)u2,
+                       (const T*)v2, (const T*)f.smoke[*sm], (T*)f.smoke[1 - *sm]);
+    *sm = 1 - *sm;
+  }
+}
+
 template <typename T>
-void step_t(const StepParams& sp, const FieldPtrs& f, int* uv, int* sm, const ProjTuning& tuning,
-            hipStream_t s) {
-  const GridView g{sp.W, sp.H, sp.P, sp.cs, f.flags};
-  T* u = (T*)f.u[*uv];
-  T* v = (T*)f.v[*uv];
-  T* smk = (T*)f.smoke[*sm];
-  const dim3 grid = cell_grid(sp.W, sp.H), block(kBX, kBY);
-  hipLaunchKernelGGL(k_forces<T>, grid, block, 0, s, sp, u, v, smk, f.p);
+void step_t(const StepParams& sp_in, const FieldPtrs& f, int* uv, int* sm,
+            const ProjTuning& tuning, hipStream_t s) {
+  const GridView g{sp_in.W, sp_in.H, sp_in.P, sp_in.cs, f.flags};
+  StepParams sp = sp_in;
+  // the first tiled projection launch applies the velocity forces (and
+  // starts the pressure sums at 0) as it loads; the smoke advection applies
+  // the smoke forces as it loads
+  const bool fuse_uv = sp.n > 0 && tuning.kernel != PROJ_COLOUR && tuning.valid(sp.pressure_on);
+  const bool fuse_smoke = sp.smoke_on;
+  sp.forces_in_projection = fuse_uv ? 1 : 0;
+  if (!fuse_uv || !fuse_smoke)
+    hipLaunchKernelGGL(k_forces<T>, cell_grid(sp.W, sp.H), dim3(kBX, kBY), 0, s, sp,
+                       fuse_uv ? 0 : 1, fuse_smoke ? 0 : 1, (T*)f.u[*uv], (T*)f.v[*uv],
+                       (T*)f.smoke[*sm], f.p);
   int cur = *uv;
   enqueue_projection<T>(sp, g, f, &cur, tuning, s);
-  u = (T*)f.u[cur];
-  v = (T*)f.v[cur];
   if (sp.pressure_on) {
     hipLaunchKernelGGL(k_minmax_partial, dim3(kReduceBlocks), dim3(256), 0, s, sp, f.p, f.scratch);
     hipLaunchKernelGGL(k_minmax_final, dim3(1), dim3(256), 0, s, f.scratch, kReduceBlocks,
                        f.prange);
   }
-  hipLaunchKernelGGL(k_extrapolate<T>, dim3((sp.W + sp.H + 255) / 256), dim3(256), 0, s, sp, u, v);
-  const int nxt = 1 - cur;
-  T* u2 = (T*)f.u[nxt];
-  T* v2 = (T*)f.v[nxt];
-  hipLaunchKernelGGL(k_advect_velocity<T>, grid, block, 0, s, sp, g, u, v, u2, v2);
-  *uv = nxt;
-  if (sp.smoke_on) {
-    T* s2 = (T*)f.smoke[1 - *sm];
-    hipLaunchKernelGGL(k_advect_smoke<T>, grid, block, 0, s, sp, g, u2, v2, smk, s2);
-    *sm = 1 - *sm;
-  }
+  if (sp.cs == 1)
+    advect_t<T, true>(sp, g, f, cur, uv, sm, s);
+  else
+    advect_t<T, false>(sp, g, f, cur, uv, sm, s);
 }
 
 template <typename T>
@@ -403,9 +454,11 @@ void enqueue_step(int storage, const StepParams& sp, const FieldPtrs& f, int* uv
   }
 }
 
-void enqueue_projection_only(int storage, const StepParams& sp, const FieldPtrs& f, int* uv,
+void enqueue_projection_only(int storage, const StepParams& sp_in, const FieldPtrs& f, int* uv,
                              const ProjTuning& tuning, hipStream_t s) {
-  const GridView g{sp.W, sp.H, sp.P, sp.cs, f.flags};
+  const GridView g{sp_in.W, sp_in.H, sp_in.P, sp_in.cs, f.flags};
+  StepParams sp = sp_in;
+  sp.forces_in_projection = 0;
   switch (storage) {
     case OSY_STORAGE_F16: enqueue_projection<__half>(sp, g, f, uv, tuning, s); break;
     case OSY_STORAGE_BF16:
