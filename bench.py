@@ -142,6 +142,9 @@ def main() -> int:
         sweep_shapes(fl, stream, handle)
         return 0
 
+    # the library's autotuner picks the projection launch shape (not timed)
+    tuning, tune_ms = fl.autotune()
+
     fl.step_async(args.warmup, stream=handle)
     torch.cuda.synchronize()
 
@@ -202,7 +205,8 @@ def main() -> int:
         "config": {"workload": f"plain_{args.workload}_{W}x{H}_n{n}", "width": W, "height": H,
                    "projection_n": n, "projection_o": round(cfg.projection_o, 4),
                    "pipe_height": cfg.pipe_height, "obstacle_radius": cfg.obstacle_radius,
-                   "seed": args.seed},
+                   "seed": args.seed, "projection_shape": tuning,
+                   "projection_ms_autotune": round(tune_ms, 4)},
         "roofline": roofline,
         "step_roofline": {"alg_bytes_per_cell_step": step_bytes_per_cell(n),
                           "achieved_GBs": round(step_alg, 1),

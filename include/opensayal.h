@@ -204,6 +204,18 @@ int osy_fluid_get_info(const osy_fluid* fluid, osy_fluid_info* info);
 int osy_fluid_set_tuning(osy_fluid* fluid, const osy_tuning* tuning);
 int osy_fluid_get_tuning(const osy_fluid* fluid, osy_tuning* tuning);
 
+/* Autotuner (this implementation's own; no counterpart in the reference):
+ * times every supported projection launch shape on this grid, projecting a
+ * copy of the handle's current u and v (the handle's fields are restored
+ * afterwards), keeps the fastest and writes it to `chosen` (may be NULL)
+ * with its time per projection in `ms` (may be NULL).  The choice is cached
+ * per (width, height, n, pressure, storage) for the process, so later
+ * handles of the same shape can adopt it with osy_fluid_autotune_cached. */
+int osy_fluid_autotune(osy_fluid* fluid, osy_tuning* chosen, double* ms);
+/* Adopt a cached autotuner choice for this handle's shape.  Returns 1 when
+ * one was found and applied, 0 when none is cached, negative on error. */
+int osy_fluid_autotune_cached(osy_fluid* fluid);
+
 /* Message of the calling thread's last failed call ("" if none). */
 const char* osy_last_error(void);
 /* OSY_ABI_VERSION of the loaded library. */

@@ -100,9 +100,27 @@ class Fluid:
         check(lib().osy_fluid_set_tuning(self._h, ctypes.byref(t)))
         self._refresh_info()
 
+    def autotune(self) -> tuple:
+        """Time every projection launch shape on this grid and keep the
+        fastest; returns (tuning dict, ms per projection)."""
+        t = OsyTuning()
+        ms = ctypes.c_double()
+        check(lib().osy_fluid_autotune(self._h, ctypes.byref(t), ctypes.byref(ms)))
+        self._refresh_info()
+        return {name: getattr(t, name) for name, _ in t._fields_}, ms.value
+
+    def autotune_cached(self) -> bool:
+        """Adopt a cached autotuner choice for this grid shape, if any."""
+        st = lib().osy_fluid_autotune_cached(self._h)
+        if st < 0:
+            check(st)
+        self._refresh_info()
+        return st == 1
+
     @property
     def projection_kernel_name(self) -> str:
-        return {0: "projection_colour", 1: "projection_tiled"}.get(self.projection_kernel, "?")
+        return {0: "projection_colour", 1: "projection_lds_tiled",
+                2: "projection_register_tiled"}.get(self.projection_kernel, "?")
 
     def launches_per_call(self, sweeps: int) -> int:
         """Projection kernel launches one ``project(sweeps)`` call makes."""

@@ -108,6 +108,9 @@ PROTOTYPES = {
     "osy_fluid_get_info": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(OsyFluidInfo)]),
     "osy_fluid_set_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(OsyTuning)]),
     "osy_fluid_get_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(OsyTuning)]),
+    "osy_fluid_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(OsyTuning),
+                                          ctypes.POINTER(ctypes.c_double)]),
+    "osy_fluid_autotune_cached": (ctypes.c_int, [ctypes.c_void_p]),
 }
 
 _lib = None

@@ -8,6 +8,9 @@
 #include <cstring>
 #include <new>
 #include <string>
+#include <map>
+#include <mutex>
+#include <tuple>
 #include <vector>
 
 #include "kernels.hpp"
@@ -465,6 +468,163 @@ int osy_fluid_get_tuning(const osy_fluid* fl, osy_tuning* t) {
   return OSY_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+using TuneKey = std::tuple<int, int, int, int, int>;  // W, H, n, pressure, storage
+std::mutex g_tune_mu;
+std::map<TuneKey, ProjTuning> g_tune_cache;
+
+TuneKey tune_key(const osy_fluid* fl) {
+  return TuneKey(fl->W, fl->H, fl->cfg.projection_n, fl->cfg.enable_pressure != 0, fl->storage);
+}
+
+// Every launch shape the autotuner times: the colour-phase kernel, the
+// register-resident kernel at 4, 8 and 16 rows per wavefront and 1..12
+// sweeps per launch, and a few LDS-tiled shapes.
+std::vector<ProjTuning> tuning_candidates(bool pressure) {
+  std::vector<ProjTuning> out;
+  ProjTuning c;
+  c.kernel = PROJ_COLOUR;
+  out.push_back(c);
+  for (int rows : {4, 8, 16})
+    for (int k : {1, 2, 3, 4, 5, 6, 7, 8, 10, 12}) {
+      ProjTuning r;
+      r.kernel = PROJ_REGS;
+      r.k = k;
+      r.tw = 128 - 4 * k;
+      r.th = kRegWaves * rows - 4 * k;
+      if (r.valid(pressure)) out.push_back(r);
+    }
+  const int lds_shapes[][3] = {{112, 16, 3}, {112, 16, 4}, {176, 24, 4}, {240, 24, 4}};
+  for (const auto& sh : lds_shapes) {
+    ProjTuning l;
+    l.kernel = PROJ_TILED;
+    l.tw = sh[0];
+    l.th = sh[1];
+    l.k = sh[2];
+    if (l.valid(pressure)) out.push_back(l);
+  }
+  return out;
+}
+
+int drop_graphs(osy_fluid* fl) {
+  for (auto& row : fl->graphs)
+    for (auto& g : row)
+      if (g.exec) {
+        OSY_HIP_CHECK(hipGraphExecDestroy(g.exec));
+        g.exec = nullptr;
+      }
+  return OSY_OK;
+}
+
+osy_tuning to_abi(const ProjTuning& t) {
+  osy_tuning o;
+  o.projection_kernel = t.kernel;
+  o.tile_width = t.tw;
+  o.tile_height = t.th;
+  o.sweeps_per_launch = t.k;
+  return o;
+}
+
+}  // namespace
+
+extern "C" {
+
+int osy_fluid_autotune(osy_fluid* fl, osy_tuning* chosen, double* ms) {
+  if (!fl) return set_error(OSY_E_INVALID, "null handle");
+  const bool pressure = fl->cfg.enable_pressure != 0;
+  StepParams sp = fl->params(fl->cfg.d_t, nullptr);
+  sp.forces_in_projection = 0;
+  double best_ms = 0.0;
+  ProjTuning best = fl->tuning;
+  if (sp.n > 0) {
+    OSY_HIP_CHECK(hipStreamSynchronize(fl->stream));
+    // snapshot of the current u, v and pressure; every trial starts from it
+    const size_t fld = (size_t)fl->P * fl->H * fl->elem;
+    const size_t pb = (size_t)fl->P * fl->H * 4;
+    void* snap = nullptr;
+    hipError_t e = hipMalloc(&snap, 2 * fld + pb);
+    if (e != hipSuccess)
+      return set_error(OSY_E_NOMEM, std::string("autotune snapshot: ") + hipGetErrorString(e));
+    const int uv0 = fl->uv;
+    char* sb = (char*)snap;
+    auto restore = [&](hipStream_t s) -> hipError_t {
+      hipError_t r = hipMemcpyAsync(fl->f.u[uv0], sb, fld, hipMemcpyDeviceToDevice, s);
+      if (r == hipSuccess)
+        r = hipMemcpyAsync(fl->f.v[uv0], sb + fld, fld, hipMemcpyDeviceToDevice, s);
+      if (r == hipSuccess)
+        r = hipMemcpyAsync(fl->f.p, sb + 2 * fld, pb, hipMemcpyDeviceToDevice, s);
+      fl->uv = uv0;
+      return r;
+    };
+    e = hipMemcpyAsync(sb, fl->f.u[uv0], fld, hipMemcpyDeviceToDevice, fl->stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(sb + fld, fl->f.v[uv0], fld, hipMemcpyDeviceToDevice, fl->stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(sb + 2 * fld, fl->f.p, pb, hipMemcpyDeviceToDevice, fl->stream);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    best_ms = 1e30;
+    for (const ProjTuning& cand : tuning_candidates(pressure)) {
+      if (e != hipSuccess) break;
+      float t_min = 1e30f;
+      for (int rep = 0; rep < 3 && e == hipSuccess; rep++) {  // rep 0 warms up
+        e = restore(fl->stream);
+        if (e == hipSuccess) e = hipEventRecord(e0, fl->stream);
+        int cur = fl->uv;
+        if (e == hipSuccess) {
+          enqueue_projection_only(fl->storage, sp, fl->f, &cur, cand, fl->stream);
+          e = hipGetLastError();
+        }
+        if (e == hipSuccess) e = hipEventRecord(e1, fl->stream);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        float t = 0.0f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+        if (rep > 0 && t < t_min) t_min = t;
+      }
+      if (e == hipSuccess && t_min < best_ms) {
+        best_ms = t_min;
+        best = cand;
+      }
+    }
+    hipError_t r = restore(fl->stream);
+    if (e == hipSuccess) e = r;
+    r = hipStreamSynchronize(fl->stream);
+    if (e == hipSuccess) e = r;
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(snap);
+    if (e != hipSuccess) return set_error(OSY_E_HIP, std::string("autotune: ") + hipGetErrorString(e));
+    int st = drop_graphs(fl);
+    if (st != OSY_OK) return st;
+    fl->tuning = best;
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    g_tune_cache[tune_key(fl)] = best;
+  }
+  if (chosen) *chosen = to_abi(fl->tuning);
+  if (ms) *ms = best_ms;
+  return OSY_OK;
+}
+
+int osy_fluid_autotune_cached(osy_fluid* fl) {
+  if (!fl) return set_error(OSY_E_INVALID, "null handle");
+  ProjTuning t;
+  {
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    auto it = g_tune_cache.find(tune_key(fl));
+    if (it == g_tune_cache.end()) return 0;
+    t = it->second;
+  }
+  OSY_HIP_CHECK(hipStreamSynchronize(fl->stream));
+  int st = drop_graphs(fl);
+  if (st != OSY_OK) return st;
+  fl->tuning = t;
+  return 1;
+}
+
 int osy_fluid_get_info(const osy_fluid* fl, osy_fluid_info* info) {
   if (!fl || !info) return set_error(OSY_E_INVALID, "null argument");
   info->width = fl->W;
@@ -474,7 +634,7 @@ int osy_fluid_get_info(const osy_fluid* fl, osy_fluid_info* info) {
   info->pitch = fl->P;
   info->storage = fl->storage;
   info->projection_kernel = fl->tuning.kernel;
-  info->sweeps_per_launch = fl->tuning.kernel == PROJ_TILED ? fl->tuning.k : 1;
+  info->sweeps_per_launch = fl->tuning.kernel == PROJ_COLOUR ? 1 : fl->tuning.k;
   info->device_bytes = fl->block_bytes;
   return OSY_OK;
 }

@@ -237,6 +237,26 @@ def test_full_size_tiled_steps_bit_exact(t):
     assert_bitwise(fl, o, f"full {TUNINGS[t]}")
 
 
+def test_autotune_keeps_state_and_results():
+    """The autotuner projects a copy of the state: the fields are unchanged
+    afterwards, and stepping with the chosen shape still matches the oracle;
+    a second handle of the same shape adopts the cached choice."""
+    cfg = make_config(300, 170, wind_speed=2.0, g=-5.0, projection_n=12,
+                      obstacle_center_x=150, obstacle_center_y=85, obstacle_radius=9.0)
+    fl, o = pair(cfg, seed=14)
+    before = {n: fl.get_field(n) for n, _ in FIELDS}
+    tuning, ms = fl.autotune()
+    assert ms > 0 and tuning["projection_kernel"] in (0, 1, 2)
+    for n, _ in FIELDS:
+        assert np.array_equal(fl.get_field(n), before[n])
+    fl.step(3)
+    o.update(steps=3)
+    assert_bitwise(fl, o, f"after autotune {tuning}")
+    other = Fluid(cfg)
+    assert other.autotune_cached()
+    assert other.tuning == tuning
+
+
 def test_invalid_tuning_rejected():
     fl = Fluid(make_config(64, 36))
     with pytest.raises(OpenSayalError):
