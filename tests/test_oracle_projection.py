@@ -143,6 +143,30 @@ def test_colour_phase_is_order_independent():
         assert np.array_equal(a.u, b.u) and np.array_equal(a.v, b.v)
 
 
+def test_divergence_norm_closed_form():
+    """One nonzero face u(i,j) = a gives divergence -a in cell (i,j) and +a
+    in cell (i-1,j): L2 = sqrt(2)|a|, max |div| = |a|; a v face likewise."""
+    c = box(12, 10)
+    o = OracleFluid(c, np.float64)
+    set_cell(o.u, 5, 4, 0.75)
+    l2, mx = o.divergence_norm()
+    assert l2 == pytest.approx(np.sqrt(2) * 0.75, rel=1e-15) and mx == 0.75
+    set_cell(o.v, 2, 6, -2.0)
+    l2, mx = o.divergence_norm()
+    assert l2 == pytest.approx(np.sqrt(2 * 0.75 ** 2 + 2 * 4.0), rel=1e-15) and mx == 2.0
+
+
+def test_pressure_range_is_min_and_max_over_all_cells():
+    """thrust::reduce min/max over all W*H pressures (solid cells hold 0)."""
+    c = box(14, 9, enable_pressure=1, projection_n=3, projection_o=1.5)
+    o = OracleFluid(c)
+    o.set_state(u=noise((9, 14), 41, 1.0), v=noise((9, 14), 42, 1.0))
+    o.update()
+    lo, hi = o.pressure_range
+    assert lo == o.p.min() and hi == o.p.max()
+    assert lo < 0 < hi
+
+
 def test_thread_count_invariance():
     c = make_config(97, 61, projection_n=30)
     outs = []
