@@ -1,0 +1,356 @@
+// projection_device.hpp - device code of the tiled projections shared by the
+// per-grid kernels (kernels_project.hip) and the batched step kernel
+// (kernels_batch.hip).
+#pragma once
+
+#include "fluid_device.hpp"
+#include "kernels.hpp"
+#include "step_device.hpp"
+
+#pragma clang fp contract(off)
+
+namespace osy {
+
+// Per-cell gate byte of the tiled kernel, built in LDS from the flags.  The
+// four face gates are set only for projected cells, so a cell that is not
+// projected changes nothing when it runs the update.
+enum : uint32_t {
+  G_L = 1u << 0,     // u(i,j) moves   (left neighbour fluid)
+  G_R = 1u << 1,     // u(i+1,j) moves (right neighbour fluid)
+  G_B = 1u << 2,     // v(i,j) moves   (bottom neighbour fluid)
+  G_T = 1u << 3,     // v(i,j+1) moves (top neighbour fluid)
+  G_UPD = 1u << 4,   // cell is projected
+  G_L2_SHIFT = 5,    // bits 5..6: log2(total_s) when total_s is 1, 2 or 4
+  G_SLOW = 1u << 7,  // projected with total_s 0 or 3: divide
+};
+
+// d = o * (div / s) for the cell with gate byte g.  For s = 1, 2, 4 the
+// quotient is div * 2^-log2(s), exact; for s = 0 or 3 (cells against a
+// wall or the obstacle) the lanes that need it divide.  Exhaustively checked
+// against IEEE division over all binary32 inputs (tests/test_quotient.py).
+__device__ inline float relaxed_quotient(float div, uint32_t g, float o) {
+  const float y = __uint_as_float(0x3F800000u - (((g >> G_L2_SHIFT) & 3u) << 23));
+  float q = div * y;
+  if (g & G_SLOW) q = div / (float)__builtin_popcount(g & 15u);
+  return o * q;
+}
+
+// x + gate(d): d when the gate bit is set, else -0 (x + -0 == x for every x,
+// signed zeros and NaN included), so the face keeps its bits when closed.
+__device__ inline float gated_add(float x, float d, uint32_t g, int bit) {
+  const uint32_t m = (uint32_t)((int32_t)(g << (31 - bit)) >> 31);
+  return x + __uint_as_float((__float_as_uint(d) & m) | (~m & 0x80000000u));
+}
+// x - gate(d): d when set, else +0 (x - +0 == x for every x).
+__device__ inline float gated_sub(float x, float d, uint32_t g, int bit) {
+  const uint32_t m = (uint32_t)((int32_t)(g << (31 - bit)) >> 31);
+  return x - __uint_as_float(__float_as_uint(d) & m);
+}
+
+// Two horizontally adjacent elements (cells 2q, 2q+1 of a row).
+template <typename T> struct PairOf;
+template <> struct PairOf<float> { using type = float2; };
+template <> struct PairOf<__half> { using type = __half2; };
+template <> struct PairOf<__hip_bfloat16> { using type = __hip_bfloat162; };
+
+template <typename T>
+__device__ inline void load_pair(const T* __restrict__ a, long k, bool in0, bool in1, float& x0,
+                                 float& x1) {
+  if (in0 && in1) {  // k is even and rows start on 64-element boundaries
+    const typename PairOf<T>::type pr = *reinterpret_cast<const typename PairOf<T>::type*>(a + k);
+    x0 = Store<T>::ld(pr.x);
+    x1 = Store<T>::ld(pr.y);
+  } else {
+    x0 = in0 ? ldf(a, k) : 0.0f;
+    x1 = in1 ? ldf(a, k + 1) : 0.0f;
+  }
+}
+
+template <typename T>
+__device__ inline void store_pair(T* __restrict__ a, long k, bool in0, bool in1, float x0,
+                                  float x1) {
+  if (in0 && in1) {
+    typename PairOf<T>::type pr;
+    pr.x = Store<T>::st(x0);
+    pr.y = Store<T>::st(x1);
+    *reinterpret_cast<typename PairOf<T>::type*>(a + k) = pr;
+  } else {
+    if (in0) stf(a, k, x0);
+    if (in1) stf(a, k + 1, x1);
+  }
+}
+
+// ---- register-resident kernel ------------------------------------------------
+//
+// Eight wavefronts stacked vertically; wavefront w holds region rows
+// [w*R, w*R + R), lane l holds cells 2l and 2l+1 of each of them: u, v and
+// the gate byte of both cells, all in VGPRs for the whole launch.  The
+// region is 128 x 8R cells and the owned tile (128 - 4k) x (8R - 4k).
+//
+// In a colour phase each row has one active cell per lane (parity b).  For
+// b = 0 every face the cell touches is in the lane's own registers.  For
+// b = 1 the right face u(2l+2) belongs to lane l+1: it is read and updated
+// through DPP lane shifts.  The top face of a wavefront's last row belongs
+// to the wavefront above: its value is published through LDS before the
+// phase and the update sent back after it (two barriers per phase).
+
+__device__ inline float lane_from_right(float x) {  // lane l gets lane l+1's x (0 for lane 63)
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x130, 0xf, 0xf, true));
+}
+__device__ inline float lane_from_left(float x) {  // lane l gets lane l-1's x (0 for lane 0)
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x138, 0xf, 0xf, true));
+}
+__device__ inline uint32_t lane_from_right_u(uint32_t x) {
+  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x130, 0xf, 0xf, true);
+}
+
+// gate of d: d when the bit is set, else +0
+__device__ inline float gate_pos(float d, uint32_t g, int bit) {
+  const uint32_t m = (uint32_t)((int32_t)(g << (31 - bit)) >> 31);
+  return __uint_as_float(__float_as_uint(d) & m);
+}
+
+// Gate byte of a cell from its flag f, the right neighbour's flag fr and
+// the top neighbour's flag ft (same encoding as the LDS-tiled kernel).
+__device__ inline uint32_t gate_byte(uint32_t f, uint32_t fr, uint32_t ft) {
+  if (!(f & CF_UPD)) return 0;
+  uint32_t g = G_UPD;
+  if (f & CF_FU) g |= G_L;
+  if (fr & CF_FU) g |= G_R;
+  if (f & CF_FV) g |= G_B;
+  if (ft & CF_FV) g |= G_T;
+  const uint32_t s = (f >> CF_S_SHIFT) & 7u;
+  if (s == 0 || s == 3)
+    g |= G_SLOW;
+  else
+    g |= (s >> 1) << G_L2_SHIFT;
+  return g;
+}
+
+struct RegArgs {
+  int tw, th, k;  // owned tile and sweeps per launch (halo 2k)
+  int ntx;        // tiles per grid row
+  int sweeps;     // sweeps this launch (<= k)
+  int forces;     // apply the velocity forces on load, pressure sums from 0
+};
+
+__device__ inline bool lc_in_tile(int lc, int K, int tw) { return lc >= 2 * K && lc < 2 * K + tw; }
+
+// update_pressure_at (src/fluid.cu:221-227): ((d * density) * cell_size) / d_t
+__device__ inline float pressure_term(float d, float density, float csf, float d_t) {
+  return ((d * density) * csf) / d_t;
+}
+
+// Register state of one lane of the register-resident kernel.
+template <int R>
+struct RegTile {
+  float u0[R], u1[R], v0[R], v1[R];  // faces of cells 2l and 2l+1, rows 0..R-1
+  float s0[R], s1[R];                // pressure sums (pressure runs only)
+  uint32_t gw[R];                    // gate bytes: cell 2l in bits 0-7, 2l+1 in 8-15
+};
+
+// apply_projection_at for the active cell of row rho (parity B) of every
+// lane.  mr masks rows outside the box, m0/m1 lanes outside it.
+template <int R, int B, bool PRESSURE>
+__device__ inline void reg_row(RegTile<R>& t, const int rho, uint32_t mr, uint32_t m0,
+                               uint32_t m1, float vt0, float vt1, float& dtop, float o,
+                               float density, float csf, float d_t) {
+  const int up = rho + 1 < R ? rho + 1 : rho;  // row above (when inside the strip)
+  if constexpr (B == 0) {
+    // active cell 2l: faces u0, u1, v0[rho], v0[rho+1] (all in this lane)
+    const uint32_t g = t.gw[rho] & m0 & mr;
+    const float uu = t.u0[rho], ur = t.u1[rho], vv = t.v0[rho];
+    const float vt = (rho + 1 < R) ? t.v0[up] : vt0;
+    const float div = ((ur - uu) + vt) - vv;
+    const float d = relaxed_quotient(div, g, o);
+    if constexpr (PRESSURE) {
+      if (g & G_UPD) t.s0[rho] = t.s0[rho] + pressure_term(d, density, csf, d_t);
+    }
+    t.u0[rho] = gated_add(uu, d, g, 0);
+    t.u1[rho] = gated_sub(ur, d, g, 1);
+    t.v0[rho] = gated_add(vv, d, g, 2);
+    if (rho + 1 < R)
+      t.v0[up] = gated_sub(vt, d, g, 3);
+    else
+      dtop = gate_pos(d, g, 3);
+  } else {
+    // active cell 2l+1: faces u1, u0 of lane l+1, v1[rho], v1[rho+1]
+    const uint32_t g = (t.gw[rho] >> 8) & m1 & mr;
+    const float uu = t.u1[rho], ur = lane_from_right(t.u0[rho]), vv = t.v1[rho];
+    const float vt = (rho + 1 < R) ? t.v1[up] : vt1;
+    const float div = ((ur - uu) + vt) - vv;
+    const float d = relaxed_quotient(div, g, o);
+    if constexpr (PRESSURE) {
+      if (g & G_UPD) t.s1[rho] = t.s1[rho] + pressure_term(d, density, csf, d_t);
+    }
+    t.u1[rho] = gated_add(uu, d, g, 0);
+    t.u0[rho] = t.u0[rho] - lane_from_left(gate_pos(d, g, 1));
+    t.v1[rho] = gated_add(vv, d, g, 2);
+    if (rho + 1 < R)
+      t.v1[up] = gated_sub(vt, d, g, 3);
+    else
+      dtop = gate_pos(d, g, 3);
+  }
+}
+
+// Rows RHO..R-1 of one phase, unrolled at compile time.
+template <int R, int PB, bool PRESSURE, int RHO>
+__device__ inline void reg_rows_from(RegTile<R>& t, int rlo, int rhi_rel, uint32_t m0, uint32_t m1,
+                                     float vt0, float vt1, float& dtop, float o, float density,
+                                     float csf, float d_t) {
+  if constexpr (RHO < R) {
+    const uint32_t mr = (RHO >= rlo && RHO < rhi_rel) ? 0xffu : 0u;  // wavefront-uniform
+    reg_row<R, ((PB + RHO) & 1), PRESSURE>(t, RHO, mr, m0, m1, vt0, vt1, dtop, o, density, csf,
+                                           d_t);
+    reg_rows_from<R, PB, PRESSURE, RHO + 1>(t, rlo, rhi_rel, m0, m1, vt0, vt1, dtop, o, density,
+                                            csf, d_t);
+  }
+}
+
+// One colour phase.  PB is the parity of the active cell in row 0 of this
+// wavefront (compile time: phases come in pairs with PB and 1 - PB).  Rows
+// outside the box of cells that still reach the tile are masked rather
+// than skipped, so the phase is straight-line code over registers.
+template <int R, int PB, bool PRESSURE>
+__device__ inline void reg_phase(RegTile<R>& t, float2 (*vrow)[64], float (*dsend)[64],
+                                 int lane, int w, int rbase, int s, int nphase, int K, int tw,
+                                 int th, float o, float density, float csf, float d_t) {
+  const int e = nphase - 1 - s;
+  const int lo = 2 * K - 1 - e;
+  const int chi = 2 * K + tw + e;
+  const int rhi = 2 * K + th + e;
+  const int lc = 2 * lane;
+  const uint32_t m0 = (lc >= lo && lc < chi) ? 0xffu : 0u;
+  const uint32_t m1 = (lc + 1 >= lo && lc + 1 < chi) ? 0xffu : 0u;
+  const int rlo = lo - rbase, rhi_rel = rhi - rbase;  // box rows of this wavefront
+  // publish row 0 for the wavefront below, read the row above
+  if (w > 0) vrow[w][lane] = make_float2(t.v0[0], t.v1[0]);
+  __syncthreads();
+  float vt0 = 0.0f, vt1 = 0.0f;
+  if (w < kRegWaves - 1) {
+    const float2 x = vrow[w + 1][lane];
+    vt0 = x.x;
+    vt1 = x.y;
+  }
+  float dtop = 0.0f;
+  if (rlo < R && rhi_rel > 0)  // wavefront-uniform: some row is in the box
+    reg_rows_from<R, PB, PRESSURE, 0>(t, rlo, rhi_rel, m0, m1, vt0, vt1, dtop, o, density, csf,
+                                      d_t);
+  if (w < kRegWaves - 1) dsend[w][lane] = dtop;
+  __syncthreads();
+  if (w > 0) {
+    // the wavefront below moved the top faces of its last row: they are
+    // this wavefront's row-0 v of parity 1 - PB
+    const float dd = dsend[w - 1][lane];
+    if constexpr (PB == 0)
+      t.v1[0] = t.v1[0] - dd;
+    else
+      t.v0[0] = t.v0[0] - dd;
+  }
+}
+
+// LDS the register-resident tile exchanges rows through.
+struct RegShared {
+  float2 vrow[kRegWaves][64];  // row 0 (v0, v1) of each wavefront
+  float dsend[kRegWaves][64];  // top-row update for the wavefront above
+};
+
+// One launch of the register-resident projection for tile `tile` (row-major
+// over the grid's ntx tiles per row), run by a workgroup of 64 * kRegWaves
+// threads.
+template <typename T, bool PRESSURE, int R>
+__device__ inline void project_regs_tile(const StepParams& sp, const uint8_t* __restrict__ flags,
+                                         const T* __restrict__ u_in, const T* __restrict__ v_in,
+                                         T* __restrict__ u_out, T* __restrict__ v_out,
+                                         float* __restrict__ p, const RegArgs& a, int tile,
+                                         RegShared& sh) {
+  float2 (*vrow)[64] = sh.vrow;
+  float (*dsend)[64] = sh.dsend;
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int K = a.k;
+  const int tx = tile % a.ntx, ty = tile / a.ntx;
+  const int x0 = tx * a.tw, y0 = ty * a.th;
+  const int gx0 = x0 - 2 * K, gy0 = y0 - 2 * K;  // region origin; gx0 is even
+  const int W = sp.W, H = sp.H, P = sp.P;
+  const int rbase = w * R;  // first region row of this wavefront
+  const int gi = gx0 + 2 * lane;
+  const bool col0 = (unsigned)gi < (unsigned)W, col1 = (unsigned)(gi + 1) < (unsigned)W;
+
+  RegTile<R> t;
+  // flags: rows rbase .. rbase+R (one above for the top faces), columns
+  // 2l, 2l+1 of this lane and 2l+2 (lane l+1's first column; lane 63 loads
+  // its own)
+  uint32_t fl0[R + 1], fl1[R + 1], fl2[R + 1];
+#pragma unroll
+  for (int rho = 0; rho <= R; rho++) {
+    const int gj = gy0 + rbase + rho;
+    const bool row_in = (unsigned)gj < (unsigned)H;
+    const long gk = (long)gj * P + gi;
+    fl0[rho] = (row_in && col0) ? flags[gk] : 0u;
+    fl1[rho] = (row_in && col1) ? flags[gk + 1] : 0u;
+    const uint32_t from_right = lane_from_right_u(fl0[rho]);
+    fl2[rho] = (lane == 63) ? ((row_in && (unsigned)(gi + 2) < (unsigned)W) ? flags[gk + 2] : 0u)
+                            : from_right;
+  }
+#pragma unroll
+  for (int rho = 0; rho < R; rho++) {
+    const int gj = gy0 + rbase + rho;
+    const bool row_in = (unsigned)gj < (unsigned)H;
+    const long gk = (long)gj * P + gi;
+    load_pair(u_in, gk, row_in && col0, row_in && col1, t.u0[rho], t.u1[rho]);
+    load_pair(v_in, gk, row_in && col0, row_in && col1, t.v0[rho], t.v1[rho]);
+    if (a.forces) {
+      if (row_in && col0) force_uv(sp, gi, gj, t.u0[rho], t.v0[rho]);
+      if (row_in && col1) force_uv(sp, gi + 1, gj, t.u1[rho], t.v1[rho]);
+    }
+    if constexpr (PRESSURE) {
+      t.s0[rho] = (row_in && col0 && !a.forces) ? p[gk] : 0.0f;
+      t.s1[rho] = (row_in && col1 && !a.forces) ? p[gk + 1] : 0.0f;
+    } else {
+      t.s0[rho] = t.s1[rho] = 0.0f;
+    }
+    t.gw[rho] = gate_byte(fl0[rho], fl1[rho], fl0[rho + 1]) |
+                (gate_byte(fl1[rho], fl2[rho], fl1[rho + 1]) << 8);
+  }
+
+  const float o = sp.o;
+  const float csf = (float)sp.cs;
+  const int nphase = 2 * a.sweeps;
+  // colour 0 (even cells) first; row 0's active parity in a colour-0 phase
+  // is (gy0 + rbase) & 1 and flips every phase
+  if (((gy0 + rbase) & 1) == 0) {
+    for (int s = 0; s < nphase; s += 2) {
+      reg_phase<R, 0, PRESSURE>(t, vrow, dsend, lane, w, rbase, s, nphase, K, a.tw, a.th, o,
+                                sp.density, csf, sp.d_t);
+      reg_phase<R, 1, PRESSURE>(t, vrow, dsend, lane, w, rbase, s + 1, nphase, K, a.tw, a.th, o,
+                                sp.density, csf, sp.d_t);
+    }
+  } else {
+    for (int s = 0; s < nphase; s += 2) {
+      reg_phase<R, 1, PRESSURE>(t, vrow, dsend, lane, w, rbase, s, nphase, K, a.tw, a.th, o,
+                                sp.density, csf, sp.d_t);
+      reg_phase<R, 0, PRESSURE>(t, vrow, dsend, lane, w, rbase, s + 1, nphase, K, a.tw, a.th, o,
+                                sp.density, csf, sp.d_t);
+    }
+  }
+
+  // write back the owned tile
+  const bool own0 = lc_in_tile(2 * lane, K, a.tw) && col0;
+  const bool own1 = lc_in_tile(2 * lane + 1, K, a.tw) && col1;
+#pragma unroll
+  for (int rho = 0; rho < R; rho++) {
+    const int r = rbase + rho;
+    const int gj = gy0 + r;
+    if (r < 2 * K || r >= 2 * K + a.th || gj >= H) continue;
+    const long gk = (long)gj * P + gi;
+    store_pair(u_out, gk, own0, own1, t.u0[rho], t.u1[rho]);
+    store_pair(v_out, gk, own0, own1, t.v0[rho], t.v1[rho]);
+    if constexpr (PRESSURE) {
+      if (own0) p[gk] = t.s0[rho];
+      if (own1) p[gk + 1] = t.s1[rho];
+    }
+  }
+}
+
+}  // namespace osy
