@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define OSY_ABI_VERSION 1
+#define OSY_ABI_VERSION 2
 
 typedef enum osy_status {
   OSY_OK = 0,
@@ -140,6 +140,25 @@ typedef struct osy_tuning {
 
 typedef struct osy_fluid osy_fluid;
 
+/* Projection launch shape of a batch: register-tile rows per wavefront (4 or
+ * 8; the tile is (128 - 4k) x (8 * rows - 4k) cells with a 2k-cell halo) and
+ * sweeps per tile k. */
+typedef struct osy_batch_tuning {
+  int32_t rows;
+  int32_t sweeps_per_tile;
+} osy_batch_tuning;
+
+typedef struct osy_batch_info {
+  int32_t count;         /* member handles                                  */
+  int32_t groups;        /* persistent launches per step (storage x pressure) */
+  int64_t work_items;    /* (stage, tile) items per step, all groups        */
+  int64_t cells;         /* sum of W*H over members                         */
+  uint64_t budget_bytes; /* 0 = unlimited                                   */
+  uint64_t used_bytes;   /* device bytes of the member handles              */
+} osy_batch_info;
+
+typedef struct osy_batch osy_batch;
+
 /* ---- configuration: replaces ConfigParser (src/config_parser.cpp:11-184) */
 
 /* Fill `cfg` with the reference's defaults (src/config_parser.cpp:21-181),
@@ -215,6 +234,43 @@ int osy_fluid_autotune(osy_fluid* fluid, osy_tuning* chosen, double* ms);
 /* Adopt a cached autotuner choice for this handle's shape.  Returns 1 when
  * one was found and applied, 0 when none is cached, negative on error. */
 int osy_fluid_autotune_cached(osy_fluid* fluid);
+
+/* Device bytes osy_fluid_create would allocate for `cfg` (for admission
+ * under a memory budget before creating). */
+int osy_fluid_device_bytes(const osy_config* cfg, uint64_t* bytes);
+
+/* ---- batches (no counterpart in the reference, which steps one Fluid) ----
+ * A batch steps many independent handles of any sizes together: one
+ * persistent kernel launch per step and storage group, replayed from a
+ * captured HIP graph.  Each member's fields after k batch steps are
+ * bit-identical to k osy_fluid_step calls on that handle alone.  While a
+ * handle is a member, osy_fluid_update/step/step_async/project/autotune and
+ * set_tuning return OSY_E_STATE; get_field, set_field, pressure_range and
+ * divergence_norm first wait for the batch's queued steps.  Each member
+ * steps with its own configuration's time.d_t. */
+
+/* `budget_bytes` bounds the sum of the members' device bytes (0 = none). */
+int osy_batch_create(uint64_t budget_bytes, osy_batch** out);
+/* Members leave the batch (they stay valid).  NULL is accepted. */
+int osy_batch_destroy(osy_batch* batch);
+/* Add a handle; OSY_E_BUDGET when its device bytes exceed what is left of
+ * the budget, OSY_E_STATE when it is already in a batch. */
+int osy_batch_add(osy_batch* batch, osy_fluid* fluid);
+int osy_batch_remove(osy_batch* batch, osy_fluid* fluid);
+/* `steps` steps of every member; synchronous.  OSY_E_STATE if a
+ * dependency wait inside the kernel timed out (results then invalid). */
+int osy_batch_step(osy_batch* batch, int32_t steps);
+/* As osy_batch_step, enqueued on `hip_stream` (NULL = the batch's stream)
+ * without waiting. */
+int osy_batch_step_async(osy_batch* batch, int32_t steps, void* hip_stream);
+int osy_batch_synchronize(osy_batch* batch);
+int osy_batch_set_tuning(osy_batch* batch, const osy_batch_tuning* tuning);
+int osy_batch_get_tuning(const osy_batch* batch, osy_batch_tuning* tuning);
+/* Autotuner: times each supported shape on the current members (their
+ * state is saved first and restored afterwards), keeps the fastest; writes
+ * it to `chosen` and its time per batch step to `ms` (either may be NULL). */
+int osy_batch_autotune(osy_batch* batch, osy_batch_tuning* chosen, double* ms);
+int osy_batch_get_info(const osy_batch* batch, osy_batch_info* info);
 
 /* Message of the calling thread's last failed call ("" if none). */
 const char* osy_last_error(void);

@@ -21,11 +21,11 @@ from typing import Optional
 
 import numpy as np
 
-from ._abi import (FIELDS, STORAGE, OpenSayalError, OsyConfig, OsyFluidInfo, OsySource,
-                   OsyTuning, check, lib)
+from ._abi import (FIELDS, STORAGE, OpenSayalError, OsyBatchInfo, OsyBatchTuning, OsyConfig,
+                   OsyFluidInfo, OsySource, OsyTuning, check, lib)
 
-__all__ = ["Config", "ConfigParser", "Fluid", "Source", "OpenSayalError", "default_config",
-           "FIELDS", "STORAGE"]
+__all__ = ["Batch", "Config", "ConfigParser", "Fluid", "Source", "OpenSayalError",
+           "default_config", "device_bytes", "FIELDS", "STORAGE"]
 
 Config = OsyConfig
 Source = OsySource
@@ -213,3 +213,82 @@ class Fluid:
         l2, mx = ctypes.c_double(), ctypes.c_double()
         check(lib().osy_fluid_divergence_norm(self._h, ctypes.byref(l2), ctypes.byref(mx)))
         return l2.value, mx.value
+
+
+def device_bytes(config: Config) -> int:
+    """Device bytes a Fluid of this configuration allocates."""
+    n = ctypes.c_uint64()
+    check(lib().osy_fluid_device_bytes(ctypes.byref(config), ctypes.byref(n)))
+    return n.value
+
+
+class Batch:
+    """Many Fluid handles stepped together by one persistent kernel launch
+    per step (per storage/pressure group), replayed from a HIP graph.  Each
+    member's fields after k batch steps are bit-identical to k ``step``
+    calls on that Fluid alone.  ``budget_bytes`` bounds the members' device
+    memory (0: none).  Members are stepped with their own config's d_t."""
+
+    def __init__(self, budget_bytes: int = 0):
+        self._h = ctypes.c_void_p()
+        check(lib().osy_batch_create(ctypes.c_uint64(budget_bytes), ctypes.byref(self._h)))
+        self.members = []
+
+    def add(self, fluid: Fluid) -> None:
+        check(lib().osy_batch_add(self._h, fluid._h))
+        self.members.append(fluid)
+
+    def remove(self, fluid: Fluid) -> None:
+        check(lib().osy_batch_remove(self._h, fluid._h))
+        self.members.remove(fluid)
+
+    def step(self, steps: int = 1) -> None:
+        check(lib().osy_batch_step(self._h, steps))
+
+    def step_async(self, steps: int = 1, stream: int = 0) -> None:
+        check(lib().osy_batch_step_async(self._h, steps, ctypes.c_void_p(stream or None)))
+
+    def synchronize(self) -> None:
+        check(lib().osy_batch_synchronize(self._h))
+
+    @property
+    def tuning(self) -> dict:
+        t = OsyBatchTuning()
+        check(lib().osy_batch_get_tuning(self._h, ctypes.byref(t)))
+        return {"rows": t.rows, "sweeps_per_tile": t.sweeps_per_tile}
+
+    def set_tuning(self, rows: int = 8, sweeps_per_tile: int = 5) -> None:
+        t = OsyBatchTuning(rows, sweeps_per_tile)
+        check(lib().osy_batch_set_tuning(self._h, ctypes.byref(t)))
+
+    def autotune(self) -> tuple:
+        """Time every supported shape on the members (state restored after);
+        keeps the fastest; returns (tuning dict, ms per batch step)."""
+        t = OsyBatchTuning()
+        ms = ctypes.c_double()
+        check(lib().osy_batch_autotune(self._h, ctypes.byref(t), ctypes.byref(ms)))
+        return {"rows": t.rows, "sweeps_per_tile": t.sweeps_per_tile}, ms.value
+
+    @property
+    def info(self) -> dict:
+        i = OsyBatchInfo()
+        check(lib().osy_batch_get_info(self._h, ctypes.byref(i)))
+        return {name: getattr(i, name) for name, _ in i._fields_}
+
+    def close(self) -> None:
+        if self._h:
+            check(lib().osy_batch_destroy(self._h))
+            self._h = ctypes.c_void_p()
+            self.members = []
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -86,6 +86,18 @@ class OsyFluidInfo(ctypes.Structure):
                 ("device_bytes", ctypes.c_size_t)]
 
 
+class OsyBatchTuning(ctypes.Structure):
+    """struct osy_batch_tuning: register-tile rows per wavefront, sweeps per tile."""
+
+    _fields_ = [("rows", _i32), ("sweeps_per_tile", _i32)]
+
+
+class OsyBatchInfo(ctypes.Structure):
+    _fields_ = [("count", _i32), ("groups", _i32), ("work_items", ctypes.c_int64),
+                ("cells", ctypes.c_int64), ("budget_bytes", ctypes.c_uint64),
+                ("used_bytes", ctypes.c_uint64)]
+
+
 # name -> (restype, argtypes); every function of include/opensayal.h.
 PROTOTYPES = {
     "osy_abi_version": (ctypes.c_int, []),
@@ -111,6 +123,20 @@ PROTOTYPES = {
     "osy_fluid_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(OsyTuning),
                                           ctypes.POINTER(ctypes.c_double)]),
     "osy_fluid_autotune_cached": (ctypes.c_int, [ctypes.c_void_p]),
+    "osy_fluid_device_bytes": (ctypes.c_int, [ctypes.POINTER(OsyConfig),
+                                              ctypes.POINTER(ctypes.c_uint64)]),
+    "osy_batch_create": (ctypes.c_int, [ctypes.c_uint64, ctypes.POINTER(ctypes.c_void_p)]),
+    "osy_batch_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "osy_batch_add": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "osy_batch_remove": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "osy_batch_step": (ctypes.c_int, [ctypes.c_void_p, _i32]),
+    "osy_batch_step_async": (ctypes.c_int, [ctypes.c_void_p, _i32, ctypes.c_void_p]),
+    "osy_batch_synchronize": (ctypes.c_int, [ctypes.c_void_p]),
+    "osy_batch_set_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(OsyBatchTuning)]),
+    "osy_batch_get_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(OsyBatchTuning)]),
+    "osy_batch_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(OsyBatchTuning),
+                                          ctypes.POINTER(ctypes.c_double)]),
+    "osy_batch_get_info": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(OsyBatchInfo)]),
 }
 
 _lib = None

@@ -25,8 +25,10 @@ SOURCES = [
     "json_value.cpp",
     "config.cpp",
     "fluid_host.cpp",
+    "batch_host.cpp",
     "kernels_step.hip",
     "kernels_project.hip",
+    "kernels_batch.hip",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -1,0 +1,197 @@
+// advect_device.hpp - per-tile bodies of the forces and the two advections,
+// shared by the per-grid kernels (kernels_step.hip) and the batched step
+// kernel (kernels_batch.hip).
+//
+// A tile is 64 columns by TY rows, run by 64 * NY threads (thread t owns
+// column t % 64 and rows t / 64 + NY * m).  The advections first stage the
+// tile plus a 2-cell halo of every field they sample in LDS with coalesced
+// loads; a back-trace that leaves the staged region reads HBM directly.
+#pragma once
+
+#include "fluid_device.hpp"
+#include "kernels.hpp"
+#include "step_device.hpp"
+
+#pragma clang fp contract(off)
+
+namespace osy {
+
+constexpr int kAdvTX = 64, kAdvHalo = 2, kAdvRX = kAdvTX + 2 * kAdvHalo;
+
+template <int TY>
+struct AdvShared {
+  static constexpr int RY = TY + 2 * kAdvHalo, N = RY * kAdvRX;
+  float su[N], sv[N], ss[N];
+  uint8_t sf[N];
+};
+
+template <int TY>
+struct AdvRegion {
+  int x0, y0;  // grid coordinates of staged cell (0, 0)
+  __device__ bool in(int i, int j) const {
+    return (unsigned)(i - x0) < (unsigned)kAdvRX &&
+           (unsigned)(j - y0) < (unsigned)AdvShared<TY>::RY;
+  }
+  __device__ int at(int i, int j) const { return (j - y0) * kAdvRX + (i - x0); }
+};
+
+// ---- P1: apply_external_forces_at (src/fluid.cu:308-349), zero_pressure,
+// over the tile at (bx, by).  Only the parts no later phase applies while
+// loading (do_uv: velocity part and pressure zeroing; do_smoke: inflow
+// smoke).
+template <typename T, int TY, int NY>
+__device__ inline void forces_tile(const StepParams& sp, int do_uv, int do_smoke, T* u, T* v,
+                                   T* smoke, float* p, int bx, int by) {
+  const int i = bx * kAdvTX + (int)(threadIdx.x & 63);
+  if (i >= sp.W) return;
+#pragma unroll
+  for (int m = 0; m < TY / NY; m++) {
+    const int j = by * TY + (int)(threadIdx.x >> 6) + NY * m;
+    if (j >= sp.H) continue;
+    const long k = (long)j * sp.P + i;
+    if (do_uv) {
+      float uu = ldf(u, k), vv = ldf(v, k);
+      force_uv(sp, i, j, uu, vv);
+      stf(u, k, uu);
+      stf(v, k, vv);
+      if (sp.pressure_on) p[k] = 0.0f;
+    }
+    if (do_smoke) {
+      const float s0 = ldf(smoke, k);
+      const float s1 = force_smoke(sp, i, j, s0);
+      if (__float_as_uint(s1) != __float_as_uint(s0)) stf(smoke, k, s1);
+    }
+  }
+}
+
+// ---- P6 + P7: apply_extrapolation then apply_velocity_advection_at
+// (src/fluid.cu:720-747, 598-612) over the tile at (bx, by).  The staged u, v
+// are the extrapolated values (u_extrapolated / v_extrapolated), which are
+// also what the HBM fallback reads.  Ends with the LDS free for reuse.
+template <typename T, bool CS1, int TY, int NY>
+__device__ inline void advect_velocity_tile(const StepParams& sp, const GridView& g,
+                                            const T* __restrict__ u, const T* __restrict__ v,
+                                            T* __restrict__ u_out, T* __restrict__ v_out, int bx,
+                                            int by, AdvShared<TY>& sh) {
+  using S = AdvShared<TY>;
+  float* su = sh.su;
+  float* sv = sh.sv;
+  uint8_t* sf = sh.sf;
+  const AdvRegion<TY> R{bx * kAdvTX - kAdvHalo, by * TY - kAdvHalo};
+  const int tid = (int)threadIdx.x;
+  for (int q = tid; q < S::N; q += 64 * NY) {
+    const int rr = q / kAdvRX, cc = q - rr * kAdvRX;
+    const int i = R.x0 + cc, j = R.y0 + rr;
+    const bool in = g.in(i, j);
+    su[q] = in ? u_extrapolated(u, g, i, j) : 0.0f;
+    sv[q] = in ? v_extrapolated(v, g, i, j) : 0.0f;
+    sf[q] = in ? g.flags[g.at(i, j)] : (uint8_t)0;
+  }
+  __syncthreads();
+  auto valid = [&](int i, int j) -> bool {
+    return R.in(i, j) ? (sf[R.at(i, j)] & CF_FLUID) != 0 : g.valid(i, j);
+  };
+  auto uval = [&](int i, int j) -> float {
+    return R.in(i, j) ? su[R.at(i, j)] : u_extrapolated(u, g, i, j);
+  };
+  auto vval = [&](int i, int j) -> float {
+    return R.in(i, j) ? sv[R.at(i, j)] : v_extrapolated(v, g, i, j);
+  };
+  const int cs = sp.cs;
+  const float dt = sp.d_t;
+  const int i = bx * kAdvTX + (tid & 63);
+#pragma unroll
+  for (int m = 0; m < TY / NY; m++) {
+    const int j = by * TY + (tid >> 6) + NY * m;
+    if (i >= sp.W || j >= sp.H) continue;
+    const int l = R.at(i, j);
+    float nu, nv;
+    {  // u face at (i*cs, (j+0.5)*cs); get_vertical_edge_velocity (:364-389)
+      const float px = (float)(CS1 ? i : i * cs);
+      const float py = CS1 ? (float)j + 0.5f : (float)(((double)j + 0.5) * (double)cs);
+      const float vel_x = su[l];
+      float avg_v = sv[l];
+      int count = 1;
+      if (valid(i - 1, j + 1)) { avg_v = avg_v + vval(i - 1, j + 1); count++; }
+      if (valid(i, j + 1)) { avg_v = avg_v + vval(i, j + 1); count++; }
+      if (valid(i - 1, j)) { avg_v = avg_v + vval(i - 1, j); count++; }
+      avg_v = div_count(avg_v, count);
+      const float x = px - vel_x * dt;
+      const float y = py - avg_v * dt;
+      nu = sample_u<CS1>(cs, x, y, valid, uval);
+    }
+    {  // v face at ((i+0.5)*cs, j*cs); get_horizontal_edge_velocity (:391-416)
+      const float px = CS1 ? (float)i + 0.5f : (float)(((double)i + 0.5) * (double)cs);
+      const float py = (float)(CS1 ? j : j * cs);
+      const float vel_y = sv[l];
+      float avg_u = su[l];
+      int count = 1;
+      if (valid(i + 1, j)) { avg_u = avg_u + uval(i + 1, j); count++; }
+      if (valid(i, j - 1)) { avg_u = avg_u + uval(i, j - 1); count++; }
+      if (valid(i + 1, j - 1)) { avg_u = avg_u + uval(i + 1, j - 1); count++; }
+      avg_u = div_count(avg_u, count);
+      const float x = px - avg_u * dt;
+      const float y = py - vel_y * dt;
+      nv = sample_v<CS1>(cs, x, y, valid, vval);
+    }
+    stf(u_out, g.at(i, j), nu);
+    stf(v_out, g.at(i, j), nv);
+  }
+  __syncthreads();
+}
+
+// ---- P8: apply_smoke_advection_at + decay_smoke_at (src/fluid.cu:560-567,
+// 758-762) over the tile at (bx, by).  The staged smoke is the smoke after
+// the forces' inflow band and source (force_smoke); the decay is applied to
+// the advected value before it is stored, which is the reference's
+// copy-back-then-decay.  Ends with the LDS free for reuse.
+template <typename T, bool CS1, int TY, int NY>
+__device__ inline void advect_smoke_tile(const StepParams& sp, const GridView& g,
+                                         const T* __restrict__ u, const T* __restrict__ v,
+                                         const T* __restrict__ smoke, T* __restrict__ smoke_out,
+                                         int bx, int by, AdvShared<TY>& sh) {
+  using S = AdvShared<TY>;
+  float* su = sh.su;
+  float* sv = sh.sv;
+  float* ss = sh.ss;
+  uint8_t* sf = sh.sf;
+  const AdvRegion<TY> R{bx * kAdvTX - kAdvHalo, by * TY - kAdvHalo};
+  const int tid = (int)threadIdx.x;
+  for (int q = tid; q < S::N; q += 64 * NY) {
+    const int rr = q / kAdvRX, cc = q - rr * kAdvRX;
+    const int i = R.x0 + cc, j = R.y0 + rr;
+    const bool in = g.in(i, j);
+    const long k = in ? g.at(i, j) : 0;
+    su[q] = in ? ldf(u, k) : 0.0f;
+    sv[q] = in ? ldf(v, k) : 0.0f;
+    ss[q] = in ? force_smoke(sp, i, j, ldf(smoke, k)) : 0.0f;
+    sf[q] = in ? g.flags[k] : (uint8_t)0;
+  }
+  __syncthreads();
+  auto valid = [&](int i, int j) -> bool {
+    return R.in(i, j) ? (sf[R.at(i, j)] & CF_FLUID) != 0 : g.valid(i, j);
+  };
+  auto uval = [&](int i, int j) -> float { return R.in(i, j) ? su[R.at(i, j)] : ldf(u, g.at(i, j)); };
+  auto vval = [&](int i, int j) -> float { return R.in(i, j) ? sv[R.at(i, j)] : ldf(v, g.at(i, j)); };
+  auto sval = [&](int i, int j) -> float {
+    return R.in(i, j) ? ss[R.at(i, j)] : force_smoke(sp, i, j, ldf(smoke, g.at(i, j)));
+  };
+  const int cs = sp.cs;
+  const int i = bx * kAdvTX + (tid & 63);
+#pragma unroll
+  for (int m = 0; m < TY / NY; m++) {
+    const int j = by * TY + (tid >> 6) + NY * m;
+    if (i >= sp.W || j >= sp.H) continue;
+    const float cx = centre<CS1>(i, cs), cy = centre<CS1>(j, cs);
+    const float vx = sample_u<CS1>(cs, cx, cy, valid, uval);
+    const float vy = sample_v<CS1>(cs, cx, cy, valid, vval);
+    const float x = cx - vx * sp.d_t;
+    const float y = cy - vy * sp.d_t;
+    float s = sample_smoke<CS1>(cs, x, y, valid, sval);
+    if (sp.decay_on) s = (float)fmax((double)(s - sp.decay_rd), 0.0);
+    stf(smoke_out, g.at(i, j), s);
+  }
+  __syncthreads();
+}
+
+}  // namespace osy

@@ -1,0 +1,84 @@
+// batch.hpp - tables and launch interface of the batched step kernel.
+//
+// A batch steps many independent simulations (of any sizes) with one
+// persistent kernel launch per step and storage group.  The host flattens
+// every simulation's step into a chain of stages (forces, projection launches
+// of up to k sweeps, velocity advection, smoke advection); each stage is cut
+// into tiles, and every (stage, tile) pair is a work item.  Items are listed
+// stage level by stage level across simulations, so every item's
+// dependencies precede it in the list.  Workgroups claim items in list order
+// from an atomic counter and wait (bounded) for the previous stage of the
+// same simulation to complete before working, so the kernel cannot deadlock
+// whatever number of workgroups is resident.
+#pragma once
+
+#include "kernels.hpp"
+
+namespace osy {
+
+enum BatchStageKind : int {
+  BK_FORCES = 0,     // forces not applied by a later load (n == 0 or smoke off)
+  BK_PROJ = 1,       // one register-tile projection launch of `sweeps` sweeps
+  BK_ADV_UV = 2,     // extrapolation + velocity advection
+  BK_ADV_SMOKE = 3,  // smoke advection + decay
+};
+
+// Tile of the forces and advection stages in a batch: 64 x 32 cells, one
+// row group per wavefront.
+constexpr int kBatchAdvTX = 64, kBatchAdvTY = 32;
+constexpr int kBatchThreads = 64 * kRegWaves;  // 512
+
+// Per simulation: parameters, buffers, and the buffer parity rule.  At launch
+// number e after the last rebuild, u and v sit in buffer uv0 ^ (e & flip_uv
+// & 1) and smoke in sm0 ^ (e & flip_sm & 1).
+struct BatchSim {
+  StepParams sp;
+  void* u[2];
+  void* v[2];
+  void* smoke[2];
+  float* p;
+  const uint8_t* flags;
+  int uv0, sm0;
+  int flip_uv, flip_sm;
+};
+
+struct BatchStage {
+  int sim;
+  int kind;
+  int q;       // BK_PROJ: projection stage index; advections: projection stage count
+  int sweeps;  // BK_PROJ
+  int forces;  // BK_PROJ: apply velocity forces on load; BK_FORCES: velocity part
+  int smoke;   // BK_FORCES: smoke part
+  int dep;     // stage that must be complete first, -1 for none
+  int dep_tiles;
+  int ntx;     // tiles per row of this stage's tiling
+};
+
+struct BatchArgs {
+  const BatchSim* sims;
+  const BatchStage* stages;
+  const uint2* items;  // (stage, tile)
+  int nitems;
+  int grid;                  // workgroups launched
+  unsigned long long* head;  // claim counter; each launch advances it by nitems + grid
+  unsigned long long* done;  // per stage: tiles completed since the last rebuild
+  int* error;                // set when a dependency wait exceeds spin_ticks
+  long long spin_ticks;      // bound of one dependency wait, wall-clock ticks
+  int tw, th, k;             // register-tile projection shape
+};
+
+// Register-tile rows per wavefront supported by the batch kernel.
+inline bool batch_rows_supported(int rows) { return rows == 4 || rows == 8; }
+
+// Occupancy-derived workgroup count for the batch kernel of this storage /
+// pressure / rows combination.
+hipError_t batch_grid_size(int storage, bool pressure, int rows, int* grid);
+
+// Enqueue one batched step (one launch).
+void enqueue_batch_step(int storage, bool pressure, int rows, const BatchArgs& a, hipStream_t s);
+
+// [min, max] of the pressure into f.prange (thrust::reduce in
+// src/fluid.cu:778-787), for simulations stepped by a batch.
+void enqueue_pressure_range(const StepParams& sp, const FieldPtrs& f, hipStream_t s);
+
+}  // namespace osy

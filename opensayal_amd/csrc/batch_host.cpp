@@ -1,0 +1,486 @@
+// batch_host.cpp - batches of solver handles stepped by one persistent kernel
+// launch per step and storage group (kernels_batch.hip).  The reference
+// steps one Fluid per process (src/main.cu:49,97); a batch runs many of them,
+// each bit-identical to stepping it alone.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "batch.hpp"
+#include "fluid_internal.hpp"
+
+using namespace osy;
+
+struct osy_batch {
+  uint64_t budget = 0, used = 0;
+  hipStream_t stream = nullptr;
+  hipStream_t last = nullptr;  // stream of the last enqueued steps
+  std::vector<osy_fluid*> members;
+  int rows = 8, k = 5;
+  bool dirty = true;
+
+  struct Group {
+    int storage = 0;
+    bool pressure = false;
+    BatchArgs args{};
+    std::vector<int> member;  // indices into members
+    std::vector<int> flip_uv, flip_sm;
+  };
+  std::vector<Group> groups;
+  void* dev = nullptr;  // every group's tables and counters, and the error word
+  int* error = nullptr;
+  hipGraphExec_t exec = nullptr;
+  long long spin_ticks = 0;
+
+  ~osy_batch() { release(); if (stream) (void)hipStreamDestroy(stream); }
+  void release() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    exec = nullptr;
+    if (dev) (void)hipFree(dev);
+    dev = nullptr;
+    error = nullptr;
+    groups.clear();
+    dirty = true;
+  }
+};
+
+namespace {
+
+struct StagePlan {
+  int kind, q, sweeps, forces, smoke, ntiles, ntx;
+};
+
+int ceil_div(int a, int b) { return (a + b - 1) / b; }
+
+// The chain of stages of one simulation's step with projection shape (tw,
+// th, k): the order the per-grid path runs its phases in (kernels_step.hip).
+std::vector<StagePlan> plan_stages(const osy_fluid* fl, const StepParams& sp, int tw, int th, int k) {
+  std::vector<StagePlan> v;
+  const int n = sp.n;
+  const int ns = n > 0 ? ceil_div(n, k) : 0;
+  const int atx = ceil_div(sp.W, kBatchAdvTX), aty = ceil_div(sp.H, kBatchAdvTY);
+  (void)fl;
+  if (n == 0 || !sp.smoke_on)
+    v.push_back({BK_FORCES, 0, 0, n == 0 ? 1 : 0, sp.smoke_on ? 0 : 1, atx * aty, atx});
+  const int ptx = ceil_div(sp.W, tw), pty = ceil_div(sp.H, th);
+  for (int q = 0; q < ns; q++) {
+    const int sweeps = std::min(k, n - q * k);
+    v.push_back({BK_PROJ, q, sweeps, q == 0 ? 1 : 0, 0, ptx * pty, ptx});
+  }
+  v.push_back({BK_ADV_UV, ns, 0, 0, 0, atx * aty, atx});
+  if (sp.smoke_on) v.push_back({BK_ADV_SMOKE, ns, 0, 0, 0, atx * aty, atx});
+  return v;
+}
+
+size_t align256(size_t x) { return (x + 255) / 256 * 256; }
+
+int sync_batch(osy_batch* b) {
+  if (b->last) OSY_HIP_CHECK(hipStreamSynchronize(b->last));
+  OSY_HIP_CHECK(hipStreamSynchronize(b->stream));
+  b->last = nullptr;
+  if (b->error) {
+    int err = 0;
+    OSY_HIP_CHECK(hipMemcpy(&err, b->error, sizeof(int), hipMemcpyDeviceToHost));
+    if (err) {
+      const int zero = 0;
+      OSY_HIP_CHECK(hipMemcpy(b->error, &zero, sizeof(int), hipMemcpyHostToDevice));
+      return set_error(OSY_E_STATE,
+                       "batch step: a stage dependency wait timed out; member results are "
+                       "invalid");
+    }
+  }
+  return OSY_OK;
+}
+
+int rebuild(osy_batch* b) {
+  b->release();
+  if (b->members.empty()) {
+    b->dirty = false;
+    return OSY_OK;
+  }
+  const int rows = b->rows, k = b->k;
+  const int tw = 128 - 4 * k, th = 8 * rows - 4 * k;
+
+  // groups by (storage, pressure), in order of first appearance
+  for (size_t m = 0; m < b->members.size(); m++) {
+    const osy_fluid* fl = b->members[m];
+    const bool pressure = fl->cfg.enable_pressure != 0;
+    auto it = std::find_if(b->groups.begin(), b->groups.end(), [&](const osy_batch::Group& g) {
+      return g.storage == fl->storage && g.pressure == pressure;
+    });
+    if (it == b->groups.end()) {
+      b->groups.emplace_back();
+      b->groups.back().storage = fl->storage;
+      b->groups.back().pressure = pressure;
+      it = b->groups.end() - 1;
+    }
+    it->member.push_back((int)m);
+  }
+
+  struct Tables {
+    std::vector<BatchSim> sims;
+    std::vector<BatchStage> stages;
+    std::vector<uint2> items;
+  };
+  std::vector<Tables> tabs(b->groups.size());
+  size_t bytes = 256;  // error word
+  for (size_t gi = 0; gi < b->groups.size(); gi++) {
+    osy_batch::Group& g = b->groups[gi];
+    Tables& t = tabs[gi];
+    std::vector<std::vector<StagePlan>> chains;
+    size_t levels = 0;
+    for (int m : g.member) {
+      osy_fluid* fl = b->members[m];
+      StepParams sp = fl->params(fl->cfg.d_t, nullptr);
+      sp.forces_in_projection = 0;
+      chains.push_back(plan_stages(fl, sp, tw, th, k));
+      levels = std::max(levels, chains.back().size());
+      BatchSim s{};
+      s.sp = sp;
+      for (int c = 0; c < 2; c++) {
+        s.u[c] = fl->f.u[c];
+        s.v[c] = fl->f.v[c];
+        s.smoke[c] = fl->f.smoke[c];
+      }
+      s.p = fl->f.p;
+      s.flags = fl->f.flags;
+      s.uv0 = fl->uv;
+      s.sm0 = fl->sm;
+      const int ns = sp.n > 0 ? ceil_div(sp.n, k) : 0;
+      s.flip_uv = (ns + 1) & 1;
+      s.flip_sm = sp.smoke_on ? 1 : 0;
+      g.flip_uv.push_back(s.flip_uv);
+      g.flip_sm.push_back(s.flip_sm);
+      t.sims.push_back(s);
+    }
+    // stages and items level by level: a stage's dependency (the previous
+    // stage of its simulation) is on the level before, so it precedes it
+    std::vector<int> prev(g.member.size(), -1), prev_tiles(g.member.size(), 0);
+    for (size_t L = 0; L < levels; L++) {
+      for (size_t si = 0; si < chains.size(); si++) {
+        if (L >= chains[si].size()) continue;
+        const StagePlan& p = chains[si][L];
+        BatchStage st{};
+        st.sim = (int)si;
+        st.kind = p.kind;
+        st.q = p.q;
+        st.sweeps = p.sweeps;
+        st.forces = p.forces;
+        st.smoke = p.smoke;
+        st.dep = prev[si];
+        st.dep_tiles = prev_tiles[si];
+        st.ntx = p.ntx;
+        const int id = (int)t.stages.size();
+        t.stages.push_back(st);
+        for (int tile = 0; tile < p.ntiles; tile++) t.items.push_back(make_uint2(id, tile));
+        prev[si] = id;
+        prev_tiles[si] = p.ntiles;
+      }
+    }
+    bytes += align256(t.sims.size() * sizeof(BatchSim)) +
+             align256(t.stages.size() * sizeof(BatchStage)) +
+             align256(t.items.size() * sizeof(uint2)) + align256(8) +
+             align256(t.stages.size() * 8);
+  }
+
+  hipError_t e = hipMalloc(&b->dev, bytes);
+  if (e != hipSuccess) {
+    b->dev = nullptr;
+    return set_error(OSY_E_NOMEM, std::string("batch tables: ") + hipGetErrorString(e));
+  }
+  OSY_HIP_CHECK(hipMemsetAsync(b->dev, 0, bytes, b->stream));
+  char* p = (char*)b->dev;
+  b->error = (int*)p;
+  p += 256;
+  for (size_t gi = 0; gi < b->groups.size(); gi++) {
+    osy_batch::Group& g = b->groups[gi];
+    Tables& t = tabs[gi];
+    BatchArgs& a = g.args;
+    a.sims = (const BatchSim*)p;
+    OSY_HIP_CHECK(hipMemcpyAsync(p, t.sims.data(), t.sims.size() * sizeof(BatchSim),
+                                 hipMemcpyHostToDevice, b->stream));
+    p += align256(t.sims.size() * sizeof(BatchSim));
+    a.stages = (const BatchStage*)p;
+    OSY_HIP_CHECK(hipMemcpyAsync(p, t.stages.data(), t.stages.size() * sizeof(BatchStage),
+                                 hipMemcpyHostToDevice, b->stream));
+    p += align256(t.stages.size() * sizeof(BatchStage));
+    a.items = (const uint2*)p;
+    OSY_HIP_CHECK(hipMemcpyAsync(p, t.items.data(), t.items.size() * sizeof(uint2),
+                                 hipMemcpyHostToDevice, b->stream));
+    p += align256(t.items.size() * sizeof(uint2));
+    a.head = (unsigned long long*)p;
+    p += align256(8);
+    a.done = (unsigned long long*)p;
+    p += align256(t.stages.size() * 8);
+    a.nitems = (int)t.items.size();
+    a.error = b->error;
+    a.spin_ticks = b->spin_ticks;
+    a.tw = tw;
+    a.th = th;
+    a.k = k;
+    OSY_HIP_CHECK(batch_grid_size(g.storage, g.pressure, rows, &a.grid));
+  }
+  OSY_HIP_CHECK(hipStreamSynchronize(b->stream));
+
+  // one step = one persistent launch per group, captured once
+  hipGraph_t graph = nullptr;
+  OSY_HIP_CHECK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  for (const osy_batch::Group& g : b->groups)
+    enqueue_batch_step(g.storage, g.pressure, rows, g.args, b->stream);
+  hipError_t le = hipGetLastError();
+  hipError_t ce = hipStreamEndCapture(b->stream, &graph);
+  if (le != hipSuccess || ce != hipSuccess) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return set_error(OSY_E_HIP, std::string("batch graph capture: ") +
+                                    hipGetErrorString(le != hipSuccess ? le : ce));
+  }
+  hipError_t ie = hipGraphInstantiate(&b->exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (ie != hipSuccess) {
+    b->exec = nullptr;
+    return set_error(OSY_E_HIP, std::string("batch graph instantiate: ") + hipGetErrorString(ie));
+  }
+  b->dirty = false;
+  return OSY_OK;
+}
+
+int enqueue_steps(osy_batch* b, int32_t steps, hipStream_t s) {
+  if (b->dirty) {
+    int st = sync_batch(b);
+    if (st == OSY_OK) st = rebuild(b);
+    if (st != OSY_OK) return st;
+  }
+  if (!b->exec || steps == 0) return OSY_OK;
+  for (int32_t i = 0; i < steps; i++) OSY_HIP_CHECK(hipGraphLaunch(b->exec, s));
+  if (s != b->stream) b->last = s;
+  if (steps & 1) {
+    for (const osy_batch::Group& g : b->groups)
+      for (size_t i = 0; i < g.member.size(); i++) {
+        osy_fluid* fl = b->members[g.member[i]];
+        fl->uv ^= g.flip_uv[i];
+        fl->sm ^= g.flip_sm[i];
+      }
+  }
+  return OSY_OK;
+}
+
+bool tuning_ok(int rows, int k) {
+  return batch_rows_supported(rows) && k >= 1 && 8 * rows - 4 * k >= 1 && 128 - 4 * k >= 2;
+}
+
+}  // namespace
+
+namespace osy {
+
+int batch_wait(osy_fluid* fl) {
+  if (!fl->batch) return OSY_OK;
+  return sync_batch(fl->batch);
+}
+
+void batch_forget(osy_fluid* fl) {
+  osy_batch* b = fl->batch;
+  if (!b) return;
+  (void)sync_batch(b);
+  b->members.erase(std::remove(b->members.begin(), b->members.end(), fl), b->members.end());
+  b->used -= fl->block_bytes;
+  fl->batch = nullptr;
+  b->release();
+}
+
+}  // namespace osy
+
+extern "C" {
+
+int osy_batch_create(uint64_t budget_bytes, osy_batch** out) {
+  if (!out) return set_error(OSY_E_INVALID, "null argument");
+  *out = nullptr;
+  osy_batch* b = new (std::nothrow) osy_batch();
+  if (!b) return set_error(OSY_E_NOMEM, "host allocation failed");
+  b->budget = budget_bytes;
+  int dev = 0, khz = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete b;
+    return set_error(OSY_E_HIP, std::string("batch create: ") + hipGetErrorString(e));
+  }
+  // one dependency wait is bounded by 2 s of wall clock
+  b->spin_ticks = (long long)(khz > 0 ? khz : 100000) * 2000;
+  *out = b;
+  return OSY_OK;
+}
+
+int osy_batch_destroy(osy_batch* b) {
+  if (!b) return OSY_OK;
+  (void)sync_batch(b);
+  for (osy_fluid* fl : b->members) fl->batch = nullptr;
+  delete b;
+  return OSY_OK;
+}
+
+int osy_batch_add(osy_batch* b, osy_fluid* fl) {
+  if (!b || !fl) return set_error(OSY_E_INVALID, "null argument");
+  if (fl->batch) return set_error(OSY_E_STATE, "the handle is already in a batch");
+  if (b->budget && b->used + fl->block_bytes > b->budget)
+    return set_error(OSY_E_BUDGET, "the handle's " + std::to_string(fl->block_bytes) +
+                                       " device bytes exceed the batch budget (" +
+                                       std::to_string(b->budget - b->used) + " bytes left)");
+  int st = sync_batch(b);
+  if (st != OSY_OK) return st;
+  OSY_HIP_CHECK(hipStreamSynchronize(fl->stream));
+  b->members.push_back(fl);
+  b->used += fl->block_bytes;
+  fl->batch = b;
+  b->release();
+  return OSY_OK;
+}
+
+int osy_batch_remove(osy_batch* b, osy_fluid* fl) {
+  if (!b || !fl) return set_error(OSY_E_INVALID, "null argument");
+  if (fl->batch != b) return set_error(OSY_E_STATE, "the handle is not in this batch");
+  int st = sync_batch(b);
+  batch_forget(fl);
+  return st;
+}
+
+int osy_batch_step(osy_batch* b, int32_t steps) {
+  int st = osy_batch_step_async(b, steps, nullptr);
+  if (st != OSY_OK) return st;
+  return sync_batch(b);
+}
+
+int osy_batch_step_async(osy_batch* b, int32_t steps, void* hip_stream) {
+  if (!b) return set_error(OSY_E_INVALID, "null handle");
+  if (steps < 0) return set_error(OSY_E_INVALID, "steps must be >= 0");
+  if (b->last && hip_stream && (hipStream_t)hip_stream != b->last) {
+    int st = sync_batch(b);
+    if (st != OSY_OK) return st;
+  }
+  return enqueue_steps(b, steps, hip_stream ? (hipStream_t)hip_stream : b->stream);
+}
+
+int osy_batch_synchronize(osy_batch* b) {
+  if (!b) return set_error(OSY_E_INVALID, "null handle");
+  return sync_batch(b);
+}
+
+int osy_batch_set_tuning(osy_batch* b, const osy_batch_tuning* t) {
+  if (!b || !t) return set_error(OSY_E_INVALID, "null argument");
+  if (!tuning_ok(t->rows, t->sweeps_per_tile))
+    return set_error(OSY_E_INVALID,
+                     "batch tuning needs rows 4 or 8 and 1 <= sweeps_per_tile < 2 * rows");
+  int st = sync_batch(b);
+  if (st != OSY_OK) return st;
+  b->rows = t->rows;
+  b->k = t->sweeps_per_tile;
+  b->release();
+  return OSY_OK;
+}
+
+int osy_batch_get_tuning(const osy_batch* b, osy_batch_tuning* t) {
+  if (!b || !t) return set_error(OSY_E_INVALID, "null argument");
+  t->rows = b->rows;
+  t->sweeps_per_tile = b->k;
+  return OSY_OK;
+}
+
+int osy_batch_autotune(osy_batch* b, osy_batch_tuning* chosen, double* ms) {
+  if (!b) return set_error(OSY_E_INVALID, "null handle");
+  int st = sync_batch(b);
+  if (st != OSY_OK) return st;
+  if (b->members.empty()) return set_error(OSY_E_STATE, "the batch has no members");
+  // save every member's device state and buffer parity
+  size_t total = 0;
+  for (osy_fluid* fl : b->members) total += fl->block_bytes;
+  void* save = nullptr;
+  hipError_t e = hipMalloc(&save, total);
+  if (e != hipSuccess)
+    return set_error(OSY_E_NOMEM, std::string("autotune snapshot: ") + hipGetErrorString(e));
+  std::vector<std::pair<int, int>> parity;
+  size_t off = 0;
+  for (osy_fluid* fl : b->members) {
+    e = hipMemcpyAsync((char*)save + off, fl->block, fl->block_bytes, hipMemcpyDeviceToDevice,
+                       b->stream);
+    if (e != hipSuccess) break;
+    off += fl->block_bytes;
+    parity.emplace_back(fl->uv, fl->sm);
+  }
+  const int rows0 = b->rows, k0 = b->k;
+  int best_rows = rows0, best_k = k0;
+  double best = 1e30;
+  hipEvent_t a = nullptr, z = nullptr;
+  if (e == hipSuccess) e = hipEventCreate(&a);
+  if (e == hipSuccess) e = hipEventCreate(&z);
+  st = e == hipSuccess ? OSY_OK : set_error(OSY_E_HIP, hipGetErrorString(e));
+  for (int rows : {4, 8}) {
+    for (int k = 1; k <= 12 && st == OSY_OK; k++) {
+      if (!tuning_ok(rows, k)) continue;
+      b->rows = rows;
+      b->k = k;
+      b->release();
+      st = enqueue_steps(b, 1, b->stream);  // warm-up (and rebuild)
+      if (st != OSY_OK) break;
+      const int reps = 3;
+      if ((e = hipEventRecord(a, b->stream)) != hipSuccess) break;
+      st = enqueue_steps(b, reps, b->stream);
+      if (st != OSY_OK) break;
+      if ((e = hipEventRecord(z, b->stream)) != hipSuccess) break;
+      st = sync_batch(b);
+      if (st != OSY_OK) break;
+      float t = 0.0f;
+      if ((e = hipEventElapsedTime(&t, a, z)) != hipSuccess) break;
+      if (t / reps < best) {
+        best = t / reps;
+        best_rows = rows;
+        best_k = k;
+      }
+    }
+  }
+  if (st == OSY_OK && e != hipSuccess) st = set_error(OSY_E_HIP, hipGetErrorString(e));
+  // restore
+  (void)hipStreamSynchronize(b->stream);
+  off = 0;
+  for (size_t i = 0; i < b->members.size(); i++) {
+    osy_fluid* fl = b->members[i];
+    if (i < parity.size()) {
+      (void)hipMemcpyAsync(fl->block, (char*)save + off, fl->block_bytes, hipMemcpyDeviceToDevice,
+                           b->stream);
+      fl->uv = parity[i].first;
+      fl->sm = parity[i].second;
+    }
+    off += fl->block_bytes;
+  }
+  (void)hipStreamSynchronize(b->stream);
+  (void)hipFree(save);
+  if (a) (void)hipEventDestroy(a);
+  if (z) (void)hipEventDestroy(z);
+  b->rows = st == OSY_OK ? best_rows : rows0;
+  b->k = st == OSY_OK ? best_k : k0;
+  b->release();
+  if (st != OSY_OK) return st;
+  if (chosen) {
+    chosen->rows = b->rows;
+    chosen->sweeps_per_tile = b->k;
+  }
+  if (ms) *ms = best;
+  return OSY_OK;
+}
+
+int osy_batch_get_info(const osy_batch* b, osy_batch_info* info) {
+  if (!b || !info) return set_error(OSY_E_INVALID, "null argument");
+  std::memset(info, 0, sizeof(*info));
+  info->count = (int32_t)b->members.size();
+  info->budget_bytes = b->budget;
+  info->used_bytes = b->used;
+  for (const osy_fluid* fl : b->members) info->cells += (int64_t)fl->W * fl->H;
+  info->groups = (int32_t)b->groups.size();
+  for (const osy_batch::Group& g : b->groups) info->work_items += g.args.nitems;
+  return OSY_OK;
+}
+
+}  // extern "C"

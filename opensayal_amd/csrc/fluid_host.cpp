@@ -13,8 +13,8 @@
 #include <tuple>
 #include <vector>
 
-#include "kernels.hpp"
-#include "osy_internal.hpp"
+#include "batch.hpp"
+#include "fluid_internal.hpp"
 #include "projection.hpp"
 
 namespace osy {
@@ -28,13 +28,6 @@ int set_error(int status, const std::string& msg) {
   return status;
 }
 void clear_error() { g_last_error.clear(); }
-
-#define OSY_HIP_CHECK(call)                                                              \
-  do {                                                                                   \
-    hipError_t e_ = (call);                                                              \
-    if (e_ != hipSuccess)                                                                \
-      return ::osy::set_error(OSY_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
-  } while (0)
 
 int config_validate(const osy_config& c) {
   if (c.width < 3 || c.height < 3 || c.width > 32768 || c.height > 32768)
@@ -75,79 +68,31 @@ int config_validate(const osy_config& c) {
 
 using namespace osy;
 
-struct osy_fluid {
-  osy_config cfg{};
-  int W = 0, H = 0, P = 0, cs = 1, storage = 0, elem = 4;
-  FieldPtrs f{};
-  int uv = 0, sm = 0;
-  hipStream_t stream = nullptr;
-  int device = 0;
-  void* block = nullptr;     // one device allocation holding every buffer
-  size_t block_bytes = 0;
-  float* staging = nullptr;  // W*H*4 bytes, reference-order copies
-  ProjTuning tuning;  // projection kernel shape, chosen at create
-  struct Graph {
-    hipGraphExec_t exec = nullptr;
-    float d_t = 0.0f;
-    int uv_out = 0, sm_out = 0;
-  } graphs[2][2];
-
-  ~osy_fluid() {
-    for (auto& row : graphs)
-      for (auto& g : row)
-        if (g.exec) (void)hipGraphExecDestroy(g.exec);
-    if (block) (void)hipFree(block);
-    if (stream) (void)hipStreamDestroy(stream);
-  }
-
-  // StepParams for this configuration and step size; `src` may be null.
-  StepParams params(float d_t, const osy_source* src) const {
-    const osy_config& c = cfg;
-    StepParams sp{};
-    sp.W = W; sp.H = H; sp.P = P; sp.cs = cs;
-    sp.n = c.projection_n;
-    sp.o = c.projection_o;
-    sp.density = c.density;
-    sp.d_t = d_t;
-    sp.smoke_length = c.smoke_length;
-    sp.band_lo = H / 2 - c.pipe_height / 2;
-    sp.band_hi = H / 2 + c.pipe_height / 2;
-    sp.smoke_count = c.smoke_count;
-    sp.smoke_height = c.smoke_height;
-    if (c.smoke_count != 1) {
-      const int spacing = (c.pipe_height - c.smoke_count * c.smoke_height) / (c.smoke_count - 1);
-      sp.smoke_mod = spacing + c.smoke_height;
-    } else {
-      sp.smoke_mod = 1;
-    }
-    sp.smoke_top = H / 2 + c.pipe_height / 2;
-    sp.sband_lo = H / 2 - c.smoke_height / 2;
-    sp.sband_hi = H / 2 + c.smoke_height / 2;
-    sp.speed = c.wind_speed;
-    sp.wind_smoke = c.wind_smoke;
-    sp.drag_on = c.drag_coeff != 0.0f;
-    // expf(-drag_coeff * d_t) (src/fluid.cu:332) is one value per step;
-    // it is evaluated once here with the host's expf.
-    sp.damping = sp.drag_on ? expf(-c.drag_coeff * d_t) : 1.0f;
-    sp.gdt = c.g * d_t;
-    sp.src_active = src && src->active;
-    if (sp.src_active) {
-      sp.src_x = src->position_x;
-      sp.src_y = src->position_y;
-      sp.src_smoke = src->smoke;
-      sp.src_velocity = src->velocity;
-    }
-    sp.pressure_on = c.enable_pressure != 0;
-    sp.smoke_on = c.enable_smoke && c.wind_smoke != 0.0f;
-    sp.decay_on = c.enable_decay != 0;
-    sp.decay_rd = c.decay_rate * d_t;
-    return sp;
-  }
-};
-
 namespace {
 
 size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+// Sizes of the parts of a handle's device allocation: six fields of the
+// storage type (u, v, smoke, double-buffered), pressure, a reference-order
+// staging buffer, the cell flags, and reduction scratch.
+struct BlockLayout {
+  size_t fld, f32, ref, flg, red, dred;
+  size_t total() const { return 6 * fld + f32 + ref + flg + red + dred; }
+};
+
+BlockLayout block_layout(int W, int H, int storage) {
+  const size_t P = align_up((size_t)W, 64);
+  const size_t cells = P * (size_t)H;
+  const size_t elem = storage == OSY_STORAGE_F32 ? 4 : 2;
+  BlockLayout L;
+  L.fld = align_up(cells * elem, 256);
+  L.f32 = align_up(cells * 4, 256);
+  L.ref = align_up((size_t)W * H * 4, 256);
+  L.flg = align_up(cells, 256);
+  L.red = align_up(sizeof(float) * (2 * kReduceBlocks + 2), 256);
+  L.dred = align_up(sizeof(double) * (2 * kReduceBlocks + 2), 256);
+  return L;
+}
 
 // Starting launch shape before any autotuning: the register-resident
 // kernel with 8 rows per wavefront and 5 sweeps per launch (a 128 x 64
@@ -189,14 +134,9 @@ int create_impl(const osy_config* cfg, osy_fluid** out) {
     return set_error(OSY_E_HIP, std::string("stream: ") + hipGetErrorString(e));
   }
 
-  const size_t cells = (size_t)fl->P * fl->H;
-  const size_t fld = align_up(cells * fl->elem, 256);
-  const size_t f32 = align_up(cells * 4, 256);
-  const size_t ref = align_up((size_t)fl->W * fl->H * 4, 256);
-  const size_t flg = align_up(cells, 256);
-  const size_t red = align_up(sizeof(float) * (2 * kReduceBlocks + 2), 256);
-  const size_t dred = align_up(sizeof(double) * (2 * kReduceBlocks + 2), 256);
-  fl->block_bytes = 6 * fld + f32 + ref + flg + red + dred;
+  const BlockLayout L = block_layout(fl->W, fl->H, fl->storage);
+  const size_t fld = L.fld, f32 = L.f32, ref = L.ref, flg = L.flg, red = L.red, dred = L.dred;
+  fl->block_bytes = L.total();
   e = hipMalloc(&fl->block, fl->block_bytes);
   if (e != hipSuccess) {
     delete fl;
@@ -291,6 +231,10 @@ int field_elem_ok(int field) { return field >= OSY_FIELD_VEL_X && field <= OSY_F
 
 }  // namespace
 
+size_t osy::fluid_block_bytes(int W, int H, int storage) {
+  return block_layout(W, H, storage).total();
+}
+
 extern "C" {
 
 int osy_abi_version(void) { return OSY_ABI_VERSION; }
@@ -328,8 +272,17 @@ int osy_fluid_create_from_json(const char* path, osy_fluid** out) {
   return osy_fluid_create(&c, out);
 }
 
+int osy_fluid_device_bytes(const osy_config* cfg, uint64_t* bytes) {
+  if (!cfg || !bytes) return set_error(OSY_E_INVALID, "null argument");
+  int st = config_validate(*cfg);
+  if (st != OSY_OK) return st;
+  *bytes = fluid_block_bytes(cfg->width, cfg->height, cfg->storage);
+  return OSY_OK;
+}
+
 int osy_fluid_destroy(osy_fluid* fluid) {
   if (!fluid) return OSY_OK;
+  if (fluid->batch) batch_forget(fluid);
   (void)hipStreamSynchronize(fluid->stream);
   delete fluid;
   return OSY_OK;
@@ -337,6 +290,7 @@ int osy_fluid_destroy(osy_fluid* fluid) {
 
 int osy_fluid_update(osy_fluid* fl, const osy_source* src, float d_t) {
   if (!fl) return set_error(OSY_E_INVALID, "null handle");
+  if (fl->batch) return set_error(OSY_E_STATE, "the handle is stepped by a batch");
   if (!std::isfinite(d_t)) return set_error(OSY_E_INVALID, "d_t must be finite");
   int st = (src && src->active) ? enqueue_eager(fl, src, d_t, fl->stream)
                                 : enqueue_graph_steps(fl, 1, d_t, fl->stream);
@@ -354,6 +308,7 @@ int osy_fluid_step(osy_fluid* fl, int32_t steps, float d_t) {
 
 int osy_fluid_step_async(osy_fluid* fl, int32_t steps, float d_t, void* hip_stream) {
   if (!fl) return set_error(OSY_E_INVALID, "null handle");
+  if (fl->batch) return set_error(OSY_E_STATE, "the handle is stepped by a batch");
   if (steps < 0) return set_error(OSY_E_INVALID, "steps must be >= 0");
   if (!std::isfinite(d_t)) return set_error(OSY_E_INVALID, "d_t must be finite");
   hipStream_t s = hip_stream ? (hipStream_t)hip_stream : fl->stream;
@@ -368,6 +323,7 @@ int osy_fluid_synchronize(osy_fluid* fl) {
 
 int osy_fluid_project(osy_fluid* fl, int32_t sweeps, void* hip_stream) {
   if (!fl) return set_error(OSY_E_INVALID, "null handle");
+  if (fl->batch) return set_error(OSY_E_STATE, "the handle is stepped by a batch");
   if (sweeps < 0) return set_error(OSY_E_INVALID, "sweeps must be >= 0");
   hipStream_t s = hip_stream ? (hipStream_t)hip_stream : fl->stream;
   StepParams sp = fl->params(fl->cfg.d_t, nullptr);
@@ -382,6 +338,7 @@ int osy_fluid_get_field(osy_fluid* fl, int32_t field, void* dst, size_t bytes) {
   if (!field_elem_ok(field)) return set_error(OSY_E_INVALID, "unknown field");
   if (bytes != (size_t)fl->W * fl->H * 4)
     return set_error(OSY_E_INVALID, "buffer must hold width*height 4-byte elements");
+  if (int st = batch_wait(fl)) return st;
   StepParams sp = fl->params(fl->cfg.d_t, nullptr);
   enqueue_export(field, fl->storage, sp, fl->f, fl->uv, fl->sm, fl->staging, fl->stream);
   OSY_HIP_CHECK(hipGetLastError());
@@ -397,6 +354,7 @@ int osy_fluid_set_field(osy_fluid* fl, int32_t field, const void* src, size_t by
   if (!field_elem_ok(field)) return set_error(OSY_E_INVALID, "unknown field");
   if (bytes != (size_t)fl->W * fl->H * 4)
     return set_error(OSY_E_INVALID, "buffer must hold width*height 4-byte elements");
+  if (int st = batch_wait(fl)) return st;
   OSY_HIP_CHECK(hipMemcpyAsync(fl->staging, src, bytes, hipMemcpyHostToDevice, fl->stream));
   StepParams sp = fl->params(fl->cfg.d_t, nullptr);
   enqueue_import(field, fl->storage, sp, fl->f, fl->uv, fl->sm, fl->staging, fl->stream);
@@ -409,6 +367,11 @@ int osy_fluid_pressure_range(osy_fluid* fl, float* min_p, float* max_p) {
   if (!fl || !min_p || !max_p) return set_error(OSY_E_INVALID, "null argument");
   float r[2] = {0.0f, 0.0f};
   if (fl->cfg.enable_pressure) {
+    if (fl->batch) {  // a batch step does not reduce; do it here
+      if (int st = batch_wait(fl)) return st;
+      enqueue_pressure_range(fl->params(fl->cfg.d_t, nullptr), fl->f, fl->stream);
+      OSY_HIP_CHECK(hipGetLastError());
+    }
     OSY_HIP_CHECK(hipMemcpyAsync(r, fl->f.prange, sizeof(r), hipMemcpyDeviceToHost, fl->stream));
     OSY_HIP_CHECK(hipStreamSynchronize(fl->stream));
   }
@@ -419,6 +382,7 @@ int osy_fluid_pressure_range(osy_fluid* fl, float* min_p, float* max_p) {
 
 int osy_fluid_divergence_norm(osy_fluid* fl, double* l2, double* linf) {
   if (!fl || !l2 || !linf) return set_error(OSY_E_INVALID, "null argument");
+  if (int st = batch_wait(fl)) return st;
   StepParams sp = fl->params(fl->cfg.d_t, nullptr);
   enqueue_divergence(fl->storage, sp, fl->f, fl->uv, fl->stream);
   OSY_HIP_CHECK(hipGetLastError());
@@ -433,6 +397,7 @@ int osy_fluid_divergence_norm(osy_fluid* fl, double* l2, double* linf) {
 
 int osy_fluid_set_tuning(osy_fluid* fl, const osy_tuning* t) {
   if (!fl || !t) return set_error(OSY_E_INVALID, "null argument");
+  if (fl->batch) return set_error(OSY_E_STATE, "the handle is stepped by a batch");
   ProjTuning pt;
   pt.kernel = t->projection_kernel;
   pt.tw = t->tile_width;
@@ -534,6 +499,7 @@ extern "C" {
 
 int osy_fluid_autotune(osy_fluid* fl, osy_tuning* chosen, double* ms) {
   if (!fl) return set_error(OSY_E_INVALID, "null handle");
+  if (fl->batch) return set_error(OSY_E_STATE, "the handle is stepped by a batch");
   const bool pressure = fl->cfg.enable_pressure != 0;
   StepParams sp = fl->params(fl->cfg.d_t, nullptr);
   sp.forces_in_projection = 0;
@@ -611,6 +577,7 @@ int osy_fluid_autotune(osy_fluid* fl, osy_tuning* chosen, double* ms) {
 
 int osy_fluid_autotune_cached(osy_fluid* fl) {
   if (!fl) return set_error(OSY_E_INVALID, "null handle");
+  if (fl->batch) return set_error(OSY_E_STATE, "the handle is stepped by a batch");
   ProjTuning t;
   {
     std::lock_guard<std::mutex> lock(g_tune_mu);

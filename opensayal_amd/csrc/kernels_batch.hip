@@ -1,0 +1,178 @@
+// kernels_batch.hip - one persistent launch steps a whole batch of
+// simulations (the per-simulation step is Fluid::update, src/fluid.cu:770-795).
+//
+// Every workgroup loops: claim the next work item from the batch's atomic
+// counter, wait until the previous stage of the same simulation is complete
+// (a per-stage counter of finished tiles, read with agent-scope acquire),
+// run the stage's body on the item's tile with the same device code as the
+// per-grid kernels, publish completion with an agent-scope release.  Items
+// are ordered so that dependencies come first; a workgroup only ever waits
+// for items already claimed by running workgroups, so the launch drains
+// whatever the occupancy.  A wait is bounded by `spin_ticks`; when the bound
+// is hit the item is skipped, the error word set, and completion still
+// published, so the launch ends in every case.
+#include "advect_device.hpp"
+#include "batch.hpp"
+#include "projection_device.hpp"
+
+#pragma clang fp contract(off)
+
+namespace osy {
+namespace {
+
+static_assert(kBatchAdvTX == kAdvTX, "advection tile width");
+
+union BatchShared {
+  RegShared reg;
+  AdvShared<kBatchAdvTY> adv;
+};
+
+// Thread 0: wait until *counter >= target.  Returns false on timeout, or
+// at once when an earlier wait of the launch timed out (so a failed launch
+// drains in about one bound rather than one per dependency level).
+__device__ inline bool wait_count(const unsigned long long* counter, unsigned long long target,
+                                  const int* error, long long spin_ticks) {
+  if (__hip_atomic_load(counter, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) >= target)
+    return true;
+  const long long t0 = wall_clock64();
+  for (;;) {
+    __builtin_amdgcn_s_sleep(2);
+    if (__hip_atomic_load(counter, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) >= target)
+      return true;
+    if (__hip_atomic_load(error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return false;
+    if (wall_clock64() - t0 > spin_ticks) return false;
+  }
+}
+
+template <typename T, bool PRESSURE, int R>
+__device__ inline void run_item(const BatchArgs& A, const BatchStage& st, const BatchSim& sim,
+                                int tile, unsigned epoch_bit, BatchShared& sh) {
+  const StepParams& sp = sim.sp;
+  const int uv = sim.uv0 ^ (int)(epoch_bit & (unsigned)sim.flip_uv);
+  const int sm = sim.sm0 ^ (int)(epoch_bit & (unsigned)sim.flip_sm);
+  const GridView g{sp.W, sp.H, sp.P, sp.cs, const_cast<uint8_t*>(sim.flags)};
+  const int kind = st.kind;
+  if (kind == BK_PROJ) {
+    RegArgs a;
+    a.tw = A.tw;
+    a.th = A.th;
+    a.k = A.k;
+    a.ntx = st.ntx;
+    a.sweeps = st.sweeps;
+    a.forces = st.forces;
+    const int in = uv ^ (st.q & 1);
+    project_regs_tile<T, PRESSURE, R>(sp, sim.flags, (const T*)sim.u[in], (const T*)sim.v[in],
+                                      (T*)sim.u[in ^ 1], (T*)sim.v[in ^ 1], sim.p, a, tile,
+                                      sh.reg);
+    return;
+  }
+  const int bx = tile % st.ntx, by = tile / st.ntx;
+  if (kind == BK_FORCES) {
+    forces_tile<T, kBatchAdvTY, kRegWaves>(sp, st.forces, st.smoke, (T*)sim.u[uv], (T*)sim.v[uv],
+                                           (T*)sim.smoke[sm], sim.p, bx, by);
+    return;
+  }
+  if (kind == BK_ADV_UV) {
+    const int in = uv ^ (st.q & 1);
+    if (sp.cs == 1)
+      advect_velocity_tile<T, true, kBatchAdvTY, kRegWaves>(
+          sp, g, (const T*)sim.u[in], (const T*)sim.v[in], (T*)sim.u[in ^ 1], (T*)sim.v[in ^ 1],
+          bx, by, sh.adv);
+    else
+      advect_velocity_tile<T, false, kBatchAdvTY, kRegWaves>(
+          sp, g, (const T*)sim.u[in], (const T*)sim.v[in], (T*)sim.u[in ^ 1], (T*)sim.v[in ^ 1],
+          bx, by, sh.adv);
+    return;
+  }
+  // BK_ADV_SMOKE: the velocities after this step's advection
+  const int cur = uv ^ ((st.q + 1) & 1);
+  if (sp.cs == 1)
+    advect_smoke_tile<T, true, kBatchAdvTY, kRegWaves>(
+        sp, g, (const T*)sim.u[cur], (const T*)sim.v[cur], (const T*)sim.smoke[sm],
+        (T*)sim.smoke[sm ^ 1], bx, by, sh.adv);
+  else
+    advect_smoke_tile<T, false, kBatchAdvTY, kRegWaves>(
+        sp, g, (const T*)sim.u[cur], (const T*)sim.v[cur], (const T*)sim.smoke[sm],
+        (T*)sim.smoke[sm ^ 1], bx, by, sh.adv);
+}
+
+template <typename T, bool PRESSURE, int R>
+__global__ void __launch_bounds__(kBatchThreads) k_batch_step(BatchArgs A) {
+  __shared__ BatchShared sh;
+  __shared__ unsigned long long s_claim;
+  __shared__ int s_ok;
+  const unsigned long long per = (unsigned long long)A.nitems + (unsigned long long)A.grid;
+  for (;;) {
+    if (threadIdx.x == 0)
+      s_claim = __hip_atomic_fetch_add(A.head, 1ULL, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    const unsigned long long raw = s_claim;
+    const unsigned long long epoch = raw / per;
+    const int idx = __builtin_amdgcn_readfirstlane((int)(raw - epoch * per));
+    if (idx >= A.nitems) break;
+    const uint2 it = A.items[idx];
+    const int sid = __builtin_amdgcn_readfirstlane((int)it.x);
+    const int tile = __builtin_amdgcn_readfirstlane((int)it.y);
+    const BatchStage st = A.stages[sid];
+    if (threadIdx.x == 0) {
+      bool ok = true;
+      if (st.dep >= 0)
+        ok = wait_count(A.done + st.dep, (epoch + 1) * (unsigned long long)st.dep_tiles,
+                        A.error, A.spin_ticks);
+      if (!ok) __hip_atomic_store(A.error, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      s_ok = ok ? 1 : 0;
+    }
+    __syncthreads();
+    // every wavefront orders its loads after thread 0's acquire
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    if (s_ok) {
+      const int simi = __builtin_amdgcn_readfirstlane(st.sim);
+      run_item<T, PRESSURE, R>(A, st, A.sims[simi], tile, (unsigned)(epoch & 1ULL), sh);
+    }
+    // every wavefront's stores are visible at agent scope before the count
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __syncthreads();
+    if (threadIdx.x == 0)
+      __hip_atomic_fetch_add(A.done + sid, 1ULL, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+template <typename T, bool PRESSURE>
+const void* batch_kernel(int rows) {
+  return rows == 4 ? (const void*)k_batch_step<T, PRESSURE, 4>
+                   : (const void*)k_batch_step<T, PRESSURE, 8>;
+}
+
+const void* batch_kernel_for(int storage, bool pressure, int rows) {
+  switch (storage) {
+    case OSY_STORAGE_F16:
+      return pressure ? batch_kernel<__half, true>(rows) : batch_kernel<__half, false>(rows);
+    case OSY_STORAGE_BF16:
+      return pressure ? batch_kernel<__hip_bfloat16, true>(rows)
+                      : batch_kernel<__hip_bfloat16, false>(rows);
+    default:
+      return pressure ? batch_kernel<float, true>(rows) : batch_kernel<float, false>(rows);
+  }
+}
+
+}  // namespace
+
+hipError_t batch_grid_size(int storage, bool pressure, int rows, int* grid) {
+  int dev = 0, cus = 0, per_cu = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (e == hipSuccess)
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_kernel_for(storage, pressure, rows),
+                                                     kBatchThreads, 0);
+  if (e != hipSuccess) return e;
+  *grid = (per_cu < 1 ? 1 : per_cu) * (cus < 1 ? 1 : cus);
+  return hipSuccess;
+}
+
+void enqueue_batch_step(int storage, bool pressure, int rows, const BatchArgs& a, hipStream_t s) {
+  const void* fn = batch_kernel_for(storage, pressure, rows);
+  void* args[] = {(void*)&a};
+  (void)hipLaunchKernel(fn, dim3(a.grid), dim3(kBatchThreads), args, 0, s);
+}
+
+}  // namespace osy

@@ -11,8 +11,10 @@
 // 569-571): same values, half the traffic.  The advections stage a tile of
 // the fields plus a 2-cell halo in LDS with coalesced loads; a back-trace
 // that leaves the staged region reads HBM directly.
+#include "advect_device.hpp"
 #include "fluid_device.hpp"
 #include "kernels.hpp"
+#include "batch.hpp"
 #include "projection.hpp"
 #include "step_device.hpp"
 
@@ -69,28 +71,11 @@ __global__ void k_build_flags(MaskParams m, uint8_t* flags) {
   flags[(long)j * m.P + i] = f;
 }
 
-// ---- P1: apply_external_forces_at (src/fluid.cu:308-349), zero_pressure ---
-// Only the parts no later kernel applies while loading: the velocity part
-// when no tiled projection launch follows, the smoke part when no smoke
-// advection follows, and the pressure zeroing in the first case.
+// ---- P1: forces over 64 x 4 tiles (forces_tile) -------------------------
 template <typename T>
-__global__ void k_forces(StepParams sp, int do_uv, int do_smoke, T* u, T* v, T* smoke, float* p) {
-  const int i = blockIdx.x * kBX + threadIdx.x;
-  const int j = blockIdx.y * kBY + threadIdx.y;
-  if (i >= sp.W || j >= sp.H) return;
-  const long k = (long)j * sp.P + i;
-  if (do_uv) {
-    float uu = ldf(u, k), vv = ldf(v, k);
-    force_uv(sp, i, j, uu, vv);
-    stf(u, k, uu);
-    stf(v, k, vv);
-    if (sp.pressure_on) p[k] = 0.0f;
-  }
-  if (do_smoke) {
-    const float s0 = ldf(smoke, k);
-    const float s1 = force_smoke(sp, i, j, s0);
-    if (__float_as_uint(s1) != __float_as_uint(s0)) stf(smoke, k, s1);
-  }
+__global__ void __launch_bounds__(256)
+    k_forces(StepParams sp, int do_uv, int do_smoke, T* u, T* v, T* smoke, float* p) {
+  forces_tile<T, kBY, kBY>(sp, do_uv, do_smoke, u, v, smoke, p, blockIdx.x, blockIdx.y);
 }
 
 // ---- P5: thrust::reduce min/max of pressure (src/fluid.cu:778-787) -------
@@ -154,141 +139,25 @@ __global__ void __launch_bounds__(256) k_minmax_final(const float* part, int npa
   }
 }
 
-// ---- staged tiles for the advections ----------------------------------------
-// A workgroup of 64 x 4 lanes advects a 64 x 16 tile (4 rows per lane).  It
-// first stages the tile plus a 2-cell halo of every field it samples in LDS;
-// samples outside the staged region fall back to HBM reads of the same
-// values.
-constexpr int kAdvTX = 64, kAdvTY = 16, kAdvHalo = 2;
-constexpr int kAdvRX = kAdvTX + 2 * kAdvHalo, kAdvRY = kAdvTY + 2 * kAdvHalo;
-constexpr int kAdvThreads = 256;
+// ---- P6-P8: the advections over 64 x 16 tiles, 256 threads ---------------
+constexpr int kAdvTY = 16, kAdvNY = 4;
 
-struct Region {
-  int x0, y0;  // grid coordinates of staged cell (0, 0)
-  __device__ bool in(int i, int j) const {
-    return (unsigned)(i - x0) < (unsigned)kAdvRX && (unsigned)(j - y0) < (unsigned)kAdvRY;
-  }
-  __device__ int at(int i, int j) const { return (j - y0) * kAdvRX + (i - x0); }
-};
-
-// ---- P6 + P7: apply_extrapolation then apply_velocity_advection_at
-// (src/fluid.cu:720-747, 598-612).  The staged u, v are the extrapolated
-// values (u_extrapolated / v_extrapolated), which are also what the HBM
-// fallback reads.
 template <typename T, bool CS1>
-__global__ void __launch_bounds__(kAdvThreads)
+__global__ void __launch_bounds__(64 * kAdvNY)
     k_advect_velocity(StepParams sp, GridView g, const T* __restrict__ u,
                       const T* __restrict__ v, T* __restrict__ u_out, T* __restrict__ v_out) {
-  __shared__ float su[kAdvRY * kAdvRX], sv[kAdvRY * kAdvRX];
-  __shared__ uint8_t sf[kAdvRY * kAdvRX];
-  const Region R{(int)blockIdx.x * kAdvTX - kAdvHalo, (int)blockIdx.y * kAdvTY - kAdvHalo};
-  const int tid = threadIdx.y * kAdvTX + threadIdx.x;
-  for (int q = tid; q < kAdvRX * kAdvRY; q += kAdvThreads) {
-    const int rr = q / kAdvRX, cc = q - rr * kAdvRX;
-    const int i = R.x0 + cc, j = R.y0 + rr;
-    const bool in = g.in(i, j);
-    su[q] = in ? u_extrapolated(u, g, i, j) : 0.0f;
-    sv[q] = in ? v_extrapolated(v, g, i, j) : 0.0f;
-    sf[q] = in ? g.flags[g.at(i, j)] : (uint8_t)0;
-  }
-  __syncthreads();
-  auto valid = [&](int i, int j) -> bool {
-    return R.in(i, j) ? (sf[R.at(i, j)] & CF_FLUID) != 0 : g.valid(i, j);
-  };
-  auto uval = [&](int i, int j) -> float {
-    return R.in(i, j) ? su[R.at(i, j)] : u_extrapolated(u, g, i, j);
-  };
-  auto vval = [&](int i, int j) -> float {
-    return R.in(i, j) ? sv[R.at(i, j)] : v_extrapolated(v, g, i, j);
-  };
-  const int cs = sp.cs;
-  const float dt = sp.d_t;
-  const int i = blockIdx.x * kAdvTX + threadIdx.x;
-#pragma unroll
-  for (int m = 0; m < kAdvTY / 4; m++) {
-    const int j = blockIdx.y * kAdvTY + threadIdx.y + 4 * m;
-    if (i >= sp.W || j >= sp.H) continue;
-    const int l = R.at(i, j);
-    float nu, nv;
-    {  // u face at (i*cs, (j+0.5)*cs); get_vertical_edge_velocity (:364-389)
-      const float px = (float)(CS1 ? i : i * cs);
-      const float py = CS1 ? (float)j + 0.5f : (float)(((double)j + 0.5) * (double)cs);
-      const float vel_x = su[l];
-      float avg_v = sv[l];
-      int count = 1;
-      if (valid(i - 1, j + 1)) { avg_v = avg_v + vval(i - 1, j + 1); count++; }
-      if (valid(i, j + 1)) { avg_v = avg_v + vval(i, j + 1); count++; }
-      if (valid(i - 1, j)) { avg_v = avg_v + vval(i - 1, j); count++; }
-      avg_v = div_count(avg_v, count);
-      const float x = px - vel_x * dt;
-      const float y = py - avg_v * dt;
-      nu = sample_u<CS1>(cs, x, y, valid, uval);
-    }
-    {  // v face at ((i+0.5)*cs, j*cs); get_horizontal_edge_velocity (:391-416)
-      const float px = CS1 ? (float)i + 0.5f : (float)(((double)i + 0.5) * (double)cs);
-      const float py = (float)(CS1 ? j : j * cs);
-      const float vel_y = sv[l];
-      float avg_u = su[l];
-      int count = 1;
-      if (valid(i + 1, j)) { avg_u = avg_u + uval(i + 1, j); count++; }
-      if (valid(i, j - 1)) { avg_u = avg_u + uval(i, j - 1); count++; }
-      if (valid(i + 1, j - 1)) { avg_u = avg_u + uval(i + 1, j - 1); count++; }
-      avg_u = div_count(avg_u, count);
-      const float x = px - avg_u * dt;
-      const float y = py - vel_y * dt;
-      nv = sample_v<CS1>(cs, x, y, valid, vval);
-    }
-    stf(u_out, g.at(i, j), nu);
-    stf(v_out, g.at(i, j), nv);
-  }
+  __shared__ AdvShared<kAdvTY> sh;
+  advect_velocity_tile<T, CS1, kAdvTY, kAdvNY>(sp, g, u, v, u_out, v_out, blockIdx.x, blockIdx.y,
+                                               sh);
 }
 
-// ---- P8: apply_smoke_advection_at + decay_smoke_at (src/fluid.cu:560-567,
-// 758-762).  The staged smoke is the smoke after the forces' inflow band and
-// source (force_smoke); the decay is applied to the advected value before it
-// is stored, which is the reference's copy-back-then-decay.
 template <typename T, bool CS1>
-__global__ void __launch_bounds__(kAdvThreads)
+__global__ void __launch_bounds__(64 * kAdvNY)
     k_advect_smoke(StepParams sp, GridView g, const T* __restrict__ u, const T* __restrict__ v,
                    const T* __restrict__ smoke, T* __restrict__ smoke_out) {
-  __shared__ float su[kAdvRY * kAdvRX], sv[kAdvRY * kAdvRX], ss[kAdvRY * kAdvRX];
-  __shared__ uint8_t sf[kAdvRY * kAdvRX];
-  const Region R{(int)blockIdx.x * kAdvTX - kAdvHalo, (int)blockIdx.y * kAdvTY - kAdvHalo};
-  const int tid = threadIdx.y * kAdvTX + threadIdx.x;
-  for (int q = tid; q < kAdvRX * kAdvRY; q += kAdvThreads) {
-    const int rr = q / kAdvRX, cc = q - rr * kAdvRX;
-    const int i = R.x0 + cc, j = R.y0 + rr;
-    const bool in = g.in(i, j);
-    const long k = in ? g.at(i, j) : 0;
-    su[q] = in ? ldf(u, k) : 0.0f;
-    sv[q] = in ? ldf(v, k) : 0.0f;
-    ss[q] = in ? force_smoke(sp, i, j, ldf(smoke, k)) : 0.0f;
-    sf[q] = in ? g.flags[k] : (uint8_t)0;
-  }
-  __syncthreads();
-  auto valid = [&](int i, int j) -> bool {
-    return R.in(i, j) ? (sf[R.at(i, j)] & CF_FLUID) != 0 : g.valid(i, j);
-  };
-  auto uval = [&](int i, int j) -> float { return R.in(i, j) ? su[R.at(i, j)] : ldf(u, g.at(i, j)); };
-  auto vval = [&](int i, int j) -> float { return R.in(i, j) ? sv[R.at(i, j)] : ldf(v, g.at(i, j)); };
-  auto sval = [&](int i, int j) -> float {
-    return R.in(i, j) ? ss[R.at(i, j)] : force_smoke(sp, i, j, ldf(smoke, g.at(i, j)));
-  };
-  const int cs = sp.cs;
-  const int i = blockIdx.x * kAdvTX + threadIdx.x;
-#pragma unroll
-  for (int m = 0; m < kAdvTY / 4; m++) {
-    const int j = blockIdx.y * kAdvTY + threadIdx.y + 4 * m;
-    if (i >= sp.W || j >= sp.H) continue;
-    const float cx = centre<CS1>(i, cs), cy = centre<CS1>(j, cs);
-    const float vx = sample_u<CS1>(cs, cx, cy, valid, uval);
-    const float vy = sample_v<CS1>(cs, cx, cy, valid, vval);
-    const float x = cx - vx * sp.d_t;
-    const float y = cy - vy * sp.d_t;
-    float s = sample_smoke<CS1>(cs, x, y, valid, sval);
-    if (sp.decay_on) s = (float)fmax((double)(s - sp.decay_rd), 0.0);
-    stf(smoke_out, g.at(i, j), s);
-  }
+  __shared__ AdvShared<kAdvTY> sh;
+  advect_smoke_tile<T, CS1, kAdvTY, kAdvNY>(sp, g, u, v, smoke, smoke_out, blockIdx.x, blockIdx.y,
+                                            sh);
 }
 
 // ---- divergence norms (Fluid::get_divergence, src/fluid.cu:192-201) -------
@@ -371,7 +240,7 @@ __global__ void k_layout(int mode, int W, int H, int P, T* dev, float* pdev, con
 template <typename T, bool CS1>
 void advect_t(const StepParams& sp, const GridView& g, const FieldPtrs& f, int cur, int* uv,
               int* sm, hipStream_t s) {
-  const dim3 grid((sp.W + kAdvTX - 1) / kAdvTX, (sp.H + kAdvTY - 1) / kAdvTY), block(kAdvTX, 4);
+  const dim3 grid((sp.W + kAdvTX - 1) / kAdvTX, (sp.H + kAdvTY - 1) / kAdvTY), block(64 * kAdvNY);
   const int nxt = 1 - cur;
   T* u2 = (T*)f.u[nxt];
   T* v2 = (T*)f.v[nxt];
@@ -397,7 +266,7 @@ void step_t(const StepParams& sp_in, const FieldPtrs& f, int* uv, int* sm,
   const bool fuse_smoke = sp.smoke_on;
   sp.forces_in_projection = fuse_uv ? 1 : 0;
   if (!fuse_uv || !fuse_smoke)
-    hipLaunchKernelGGL(k_forces<T>, cell_grid(sp.W, sp.H), dim3(kBX, kBY), 0, s, sp,
+    hipLaunchKernelGGL(k_forces<T>, cell_grid(sp.W, sp.H), dim3(kBX * kBY), 0, s, sp,
                        fuse_uv ? 0 : 1, fuse_smoke ? 0 : 1, (T*)f.u[*uv], (T*)f.v[*uv],
                        (T*)f.smoke[*sm], f.p);
   int cur = *uv;
@@ -466,6 +335,11 @@ void enqueue_projection_only(int storage, const StepParams& sp_in, const FieldPt
       break;
     default: enqueue_projection<float>(sp, g, f, uv, tuning, s); break;
   }
+}
+
+void enqueue_pressure_range(const StepParams& sp, const FieldPtrs& f, hipStream_t s) {
+  hipLaunchKernelGGL(k_minmax_partial, dim3(kReduceBlocks), dim3(256), 0, s, sp, f.p, f.scratch);
+  hipLaunchKernelGGL(k_minmax_final, dim3(1), dim3(256), 0, s, f.scratch, kReduceBlocks, f.prange);
 }
 
 void enqueue_divergence(int storage, const StepParams& sp, const FieldPtrs& f, int uv,

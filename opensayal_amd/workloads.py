@@ -122,6 +122,36 @@ def batched_specs(count: int = 256, seed: int = 2024, steps: int = 500):
     return out
 
 
+def small_specs(count: int = 256, seed: int = 99, steps: int = 200):
+    """Mixed small grids: width from {64, 96, ..., 384}, height = round(9W/16),
+    otherwise drawn as in ``batched_specs``."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for k in range(count):
+        W = int(rng.integers(0, 11)) * 32 + 64
+        H = int(round(W * 9 / 16))
+        cx = int(round(rng.uniform(0.2, 0.8) * W))
+        cy = int(round(rng.uniform(0.2, 0.8) * H))
+        r = float(rng.uniform(0.02, 0.06) * W)
+        speed = float(rng.uniform(0.5, 3.0))
+        n = int(rng.choice([20, 50, 100]))
+        cfg = wind_tunnel(W, H, pipe_height=H // 4, cx=cx, cy=cy, radius=r, speed=speed, n=n)
+        out.append(SimSpec(cfg, seed * 1000003 + k, steps))
+    return out
+
+
+def start(spec: SimSpec):
+    """A Fluid of ``spec.cfg`` holding ``initial_state(spec.cfg, spec.seed)``."""
+    from . import Fluid
+
+    fl = Fluid(spec.cfg)
+    u, v, s = initial_state(spec.cfg, spec.seed)
+    fl.set_field("vel_x", u)
+    fl.set_field("vel_y", v)
+    fl.set_field("smoke", s)
+    return fl
+
+
 def serving_specs(count: int, seed: int = 7, rate_hz: float = 200.0):
     """Poisson arrivals of requests: width log-uniform in [64, 2048] (16:9),
     steps uniform in [50, 2000], n from {20, 50, 100}, obstacle and inflow as
