@@ -74,6 +74,78 @@ def cpu_baseline(cfg, seed: int, budget_s: float = 12.0) -> dict:
     }
 
 
+def pct(xs, q):
+    import numpy as np
+
+    return float(np.percentile(np.asarray(xs, dtype=np.float64), q))
+
+
+def batch_leg(specs, steps: int, warmup: int, plain_steps: int = 3, autotune: bool = True) -> dict:
+    """Steps every simulation of `specs` together as one Batch (one persistent
+    launch per step, replayed from a HIP graph), timing each step with HIP
+    events on the batch's stream; then the same simulations one after another
+    through the per-grid path (each its own captured graph) for comparison."""
+    import torch
+
+    from opensayal_amd import Batch, workloads as wl
+
+    t0 = time.perf_counter()
+    fluids = [wl.start(sp) for sp in specs]
+    setup_s = time.perf_counter() - t0
+    cells = sum(f.width * f.height for f in fluids)
+    stream = torch.cuda.Stream()
+    h = stream.cuda_stream
+    b = Batch()
+    for f in fluids:
+        b.add(f)
+    tuning, tune_ms = b.autotune() if autotune else (b.tuning, None)
+    b.step_async(warmup, stream=h)
+    b.synchronize()
+    evs = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
+    torch.cuda.synchronize()
+    evs[0].record(stream)
+    for k in range(steps):
+        b.step_async(1, stream=h)
+        evs[k + 1].record(stream)
+    torch.cuda.synchronize()
+    b.synchronize()
+    per = [evs[k].elapsed_time(evs[k + 1]) for k in range(steps)]
+    total_ms = evs[0].elapsed_time(evs[-1])
+    info = b.info
+    b.close()
+
+    # the per-grid path: each simulation alone, sequentially on one stream
+    for f in fluids:
+        f.step_async(1, stream=h)  # capture its graphs (untimed)
+    torch.cuda.synchronize()
+    a, z = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record(stream)
+    for _ in range(plain_steps):
+        for f in fluids:
+            f.step_async(1, stream=h)
+    z.record(stream)
+    torch.cuda.synchronize()
+    plain_ms = a.elapsed_time(z) / plain_steps
+    for f in fluids:
+        f.close()
+    value = cells * steps / (total_ms / 1e3) / 1e9
+    plain = cells / (plain_ms / 1e3) / 1e9
+    ns = sorted(sp.cfg.projection_n for sp in specs)
+    return {
+        "sims": len(specs), "cells_per_step": cells, "steps": steps, "warmup": warmup,
+        "n_values": sorted(set(ns)),
+        "value": round(value, 4), "unit": "Gcell-steps/s",
+        "ms_per_step": round(total_ms / steps, 4),
+        "step_ms_p50": round(pct(per, 50), 4), "step_ms_p99": round(pct(per, 99), 4),
+        "tuning": tuning, "autotune_ms_per_step": None if tune_ms is None else round(tune_ms, 4),
+        "work_items_per_step": info["work_items"], "groups": info["groups"],
+        "plain_sequential": {"value": round(plain, 4), "ms_per_step": round(plain_ms, 4),
+                             "steps": plain_steps},
+        "speedup_vs_plain": round(value / plain, 3),
+        "setup_s": round(setup_s, 2),
+    }
+
+
 def sweep_shapes(fl, stream, handle, steps: int = 40) -> None:
     """Per-step time for a grid of projection launch shapes (one JSON line each)."""
     import torch
@@ -110,10 +182,18 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=300)
     ap.add_argument("--warmup", type=int, default=30)
-    ap.add_argument("--workload", default="full", choices=["full", "small"])
+    ap.add_argument("--workload", default="full",
+                    choices=["full", "small", "batched", "mixed_small", "serving"])
+    ap.add_argument("--batch-steps", type=int, default=20,
+                    help="timed batch steps of the batched legs of the default run")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched legs")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--kernel-reps", type=int, default=20)
+    ap.add_argument("--requests", type=int, default=64, help="serving: number of requests")
+    ap.add_argument("--rate", type=float, default=8.0, help="serving: arrivals per second")
+    ap.add_argument("--budget-gb", type=float, default=200.0, help="serving: HBM state budget")
+    ap.add_argument("--chunk", type=int, default=16, help="serving: steps between admissions")
     ap.add_argument("--sweep", action="store_true",
                     help="time a grid of projection launch shapes instead")
     args = ap.parse_args()
@@ -126,6 +206,18 @@ def main() -> int:
     import torch
 
     from opensayal_amd import Fluid, workloads as wl
+
+    if args.workload in ("batched", "mixed_small"):
+        specs = wl.batched_specs() if args.workload == "batched" else wl.small_specs()
+        out = batch_leg(specs, args.steps, args.warmup)
+        out.update({"metric": METRIC, "n_gpus": 1, "higher_is_better": True,
+                    "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+                    "data": "synthetic: seeded +-0.01 velocity noise, smoke 1 on the inflow",
+                    "config": {"workload": args.workload}})
+        print(json.dumps(out), flush=True)
+        return 0
+    if args.workload == "serving":
+        return serving_main(args)
 
     cfg = wl.plain_full() if args.workload == "full" else wl.plain_small()
     W, H, n = cfg.width, cfg.height, cfg.projection_n
@@ -213,12 +305,40 @@ def main() -> int:
                           "frac": round(step_alg / HBM_PEAK_GBS, 4)},
         "wall_s": round(wall, 3),
     }
+    fl.close()
+    if not args.no_batch and args.workload == "full":
+        # 256 uneven grids (128..1024 wide, n from {20, 50, 100}) and 256
+        # small ones (64..384 wide), each as one batch against the per-grid path
+        out["batched"] = batch_leg(wl.batched_specs(), args.batch_steps, 3)
+        out["batched_mixed_small"] = batch_leg(wl.small_specs(), args.batch_steps * 5, 5)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(cfg, args.seed)
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)
-    fl.close()
+    return 0
+
+
+def serving_main(args) -> int:
+    """Poisson arrivals of mixed requests served by continuous batching."""
+    from opensayal_amd import workloads as wl
+    from opensayal_amd.serving import DeviceBackend, Server
+
+    arrivals = wl.serving_specs(count=args.requests, seed=args.seed, rate_hz=args.rate)
+    backend = DeviceBackend()
+    try:
+        rep = Server(backend, budget_bytes=int(args.budget_gb * 1e9),
+                     max_chunk=args.chunk).run(arrivals)
+    finally:
+        backend.close()
+    s = rep.summary()
+    out = {"metric": METRIC, "value": round(s["gcell_steps_s_wall"], 4), "unit": "Gcell-steps/s",
+           "n_gpus": 1, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32/f16 storage", "data": "synthetic requests: Poisson arrivals",
+           "config": {"workload": "serving", "requests": args.requests, "rate_hz": args.rate,
+                      "budget_gb": args.budget_gb, "max_chunk": args.chunk, "seed": args.seed},
+           "serving": {k: (round(v, 5) if isinstance(v, float) else v) for k, v in s.items()}}
+    print(json.dumps(out), flush=True)
     return 0
 
 
