@@ -194,6 +194,9 @@ def main() -> int:
     ap.add_argument("--rate", type=float, default=8.0, help="serving: arrivals per second")
     ap.add_argument("--budget-gb", type=float, default=200.0, help="serving: HBM state budget")
     ap.add_argument("--chunk", type=int, default=16, help="serving: steps between admissions")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run only device-to-device copies of a known size (to scale "
+                         "FETCH_SIZE / WRITE_SIZE counters)")
     ap.add_argument("--sweep", action="store_true",
                     help="time a grid of projection launch shapes instead")
     args = ap.parse_args()
@@ -206,6 +209,17 @@ def main() -> int:
     import torch
 
     from opensayal_amd import Fluid, workloads as wl
+
+    if args.calibrate_copy:
+        n = 128 * 1024 * 1024  # floats: 512 MiB read and 512 MiB written per copy
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_copy": {"bytes_read": 4 * n, "bytes_written": 4 * n,
+                                             "copies": 5}}), flush=True)
+        return 0
 
     if args.workload in ("batched", "mixed_small"):
         specs = wl.batched_specs() if args.workload == "batched" else wl.small_specs()

@@ -96,17 +96,22 @@ __device__ inline void run_item(const BatchArgs& A, const BatchStage& st, const 
         (T*)sim.smoke[sm ^ 1], bx, by, sh.adv);
 }
 
+// The workgroup's loop keeps every thread-0 region between two barriers of
+// the same iteration: the claim of the next item shares one region with the
+// completion count of the current one.  (Two thread-0 regions adjacent across
+// the loop's back-edge let the compiler merge them and move a barrier into a
+// loop that thread 0 leaves, which deadlocks the other threads.)
 template <typename T, bool PRESSURE, int R>
 __global__ void __launch_bounds__(kBatchThreads) k_batch_step(BatchArgs A) {
   __shared__ BatchShared sh;
   __shared__ unsigned long long s_claim;
   __shared__ int s_ok;
   const unsigned long long per = (unsigned long long)A.nitems + (unsigned long long)A.grid;
+  if (threadIdx.x == 0)
+    s_claim = __hip_atomic_fetch_add(A.head, 1ULL, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __syncthreads();
+  unsigned long long raw = s_claim;
   for (;;) {
-    if (threadIdx.x == 0)
-      s_claim = __hip_atomic_fetch_add(A.head, 1ULL, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __syncthreads();
-    const unsigned long long raw = s_claim;
     const unsigned long long epoch = raw / per;
     const int idx = __builtin_amdgcn_readfirstlane((int)(raw - epoch * per));
     if (idx >= A.nitems) break;
@@ -125,15 +130,19 @@ __global__ void __launch_bounds__(kBatchThreads) k_batch_step(BatchArgs A) {
     __syncthreads();
     // every wavefront orders its loads after thread 0's acquire
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    if (s_ok) {
+    if (__builtin_amdgcn_readfirstlane(s_ok)) {
       const int simi = __builtin_amdgcn_readfirstlane(st.sim);
       run_item<T, PRESSURE, R>(A, st, A.sims[simi], tile, (unsigned)(epoch & 1ULL), sh);
     }
     // every wavefront's stores are visible at agent scope before the count
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     __syncthreads();
-    if (threadIdx.x == 0)
+    if (threadIdx.x == 0) {
       __hip_atomic_fetch_add(A.done + sid, 1ULL, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      s_claim = __hip_atomic_fetch_add(A.head, 1ULL, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();
+    raw = s_claim;
   }
 }
 
