@@ -83,49 +83,45 @@ def pct(xs, q):
 def batch_leg(specs, steps: int, warmup: int, plain_steps: int = 3, autotune: bool = True) -> dict:
     """Steps every simulation of `specs` together as one Batch (one persistent
     launch per step, replayed from a HIP graph), timing each step with HIP
-    events on the batch's stream; then the same simulations one after another
-    through the per-grid path (each its own captured graph) for comparison."""
-    import torch
-
-    from opensayal_amd import Batch, workloads as wl
+    events on the stream the steps run on; then the same simulations one
+    after another through the per-grid path (each its own captured graph)
+    for comparison."""
+    from opensayal_amd import Batch, Event, Stream, workloads as wl
 
     t0 = time.perf_counter()
     fluids = [wl.start(sp) for sp in specs]
     setup_s = time.perf_counter() - t0
     cells = sum(f.width * f.height for f in fluids)
-    stream = torch.cuda.Stream()
-    h = stream.cuda_stream
+    stream = Stream()
     b = Batch()
     for f in fluids:
         b.add(f)
     tuning, tune_ms = b.autotune() if autotune else (b.tuning, None)
-    b.step_async(warmup, stream=h)
+    b.step_async(warmup, stream=stream)
     b.synchronize()
-    evs = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
-    torch.cuda.synchronize()
+    evs = [Event() for _ in range(steps + 1)]
     evs[0].record(stream)
     for k in range(steps):
-        b.step_async(1, stream=h)
+        b.step_async(1, stream=stream)
         evs[k + 1].record(stream)
-    torch.cuda.synchronize()
     b.synchronize()
-    per = [evs[k].elapsed_time(evs[k + 1]) for k in range(steps)]
-    total_ms = evs[0].elapsed_time(evs[-1])
+    per = [evs[k].elapsed_ms(evs[k + 1]) for k in range(steps)]
+    total_ms = evs[0].elapsed_ms(evs[-1])
     info = b.info
     b.close()
 
     # the per-grid path: each simulation alone, sequentially on one stream
     for f in fluids:
-        f.step_async(1, stream=h)  # capture its graphs (untimed)
-    torch.cuda.synchronize()
-    a, z = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        f.step_async(1, stream=stream)  # capture its graphs (untimed)
+    stream.synchronize()
+    a, z = Event(), Event()
     a.record(stream)
     for _ in range(plain_steps):
         for f in fluids:
-            f.step_async(1, stream=h)
+            f.step_async(1, stream=stream)
     z.record(stream)
-    torch.cuda.synchronize()
-    plain_ms = a.elapsed_time(z) / plain_steps
+    stream.synchronize()
+    plain_ms = a.elapsed_ms(z) / plain_steps
     for f in fluids:
         f.close()
     value = cells * steps / (total_ms / 1e3) / 1e9
@@ -146,9 +142,9 @@ def batch_leg(specs, steps: int, warmup: int, plain_steps: int = 3, autotune: bo
     }
 
 
-def sweep_shapes(fl, stream, handle, steps: int = 40) -> None:
+def sweep_shapes(fl, stream, steps: int = 40) -> None:
     """Per-step time for a grid of projection launch shapes (one JSON line each)."""
-    import torch
+    from opensayal_amd import Event
 
     shapes = [dict(projection_kernel=0)]
     for tw in (112, 176, 240):
@@ -166,13 +162,13 @@ def sweep_shapes(fl, stream, handle, steps: int = 40) -> None:
             fl.set_tuning(**sh)
         except Exception:
             continue
-        fl.step_async(3, stream=handle)
-        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        fl.step_async(3, stream=stream)
+        a, b = Event(), Event()
         a.record(stream)
-        fl.step_async(steps, stream=handle)
+        fl.step_async(steps, stream=stream)
         b.record(stream)
-        torch.cuda.synchronize()
-        ms = a.elapsed_time(b) / steps
+        stream.synchronize()
+        ms = a.elapsed_ms(b) / steps
         print(json.dumps({"shape": sh, "ms_per_step": round(ms, 4),
                           "gcell_steps_s": round(fl.width * fl.height / ms / 1e6, 3)}), flush=True)
 
@@ -206,19 +202,15 @@ def main() -> int:
               "device (multi-GPU sharding is described in DESIGN.md, not built)", file=sys.stderr)
         return 2
 
-    import torch
-
-    from opensayal_amd import Fluid, workloads as wl
+    from opensayal_amd import Event, Fluid, Stream, lib as _lib, workloads as wl
 
     if args.calibrate_copy:
-        n = 128 * 1024 * 1024  # floats: 512 MiB read and 512 MiB written per copy
-        src = torch.ones(n, dtype=torch.float32, device="cuda")
-        dst = torch.empty_like(src)
-        for _ in range(5):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        print(json.dumps({"calibrate_copy": {"bytes_read": 4 * n, "bytes_written": 4 * n,
-                                             "copies": 5}}), flush=True)
+        nbytes = 512 * 1024 * 1024  # read and written per copy
+        from opensayal_amd._abi import check
+
+        check(_lib().osy_calibration_copy(nbytes, 5))
+        print(json.dumps({"calibrate_copy": {"kernel": "k_copy", "bytes_read": nbytes,
+                                             "bytes_written": nbytes, "copies": 5}}), flush=True)
         return 0
 
     if args.workload in ("batched", "mixed_small"):
@@ -241,29 +233,27 @@ def main() -> int:
     fl.set_field("vel_y", v)
     fl.set_field("smoke", s)
 
-    stream = torch.cuda.Stream()
-    handle = stream.cuda_stream
+    stream = Stream()
 
     if args.sweep:
-        sweep_shapes(fl, stream, handle)
+        sweep_shapes(fl, stream)
         return 0
 
     # the library's autotuner picks the projection launch shape (not timed)
     tuning, tune_ms = fl.autotune()
 
-    fl.step_async(args.warmup, stream=handle)
-    torch.cuda.synchronize()
+    fl.step_async(args.warmup, stream=stream)
+    stream.synchronize()
 
-    # timed region: exactly K steps
-    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
+    # timed region: exactly K steps, HIP events on the stream they run on
+    start, end = Event(), Event()
     t0 = time.perf_counter()
     start.record(stream)
-    fl.step_async(args.steps, stream=handle)
+    fl.step_async(args.steps, stream=stream)
     end.record(stream)
-    torch.cuda.synchronize()
+    stream.synchronize()
     wall = time.perf_counter() - t0
-    gpu_ms = start.elapsed_time(end)
+    gpu_ms = start.elapsed_ms(end)
     ms_per_step = gpu_ms / args.steps
     value = W * H * args.steps / (gpu_ms / 1e3) / 1e9
 
@@ -272,13 +262,13 @@ def main() -> int:
     spl = fl.sweeps_per_launch
     durations = []
     for _ in range(args.kernel_reps):
-        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a, b = Event(), Event()
         a.record(stream)
-        fl.project(sweeps=spl, stream=handle, sync=False)
+        fl.project(sweeps=spl, stream=stream, sync=False)
         b.record(stream)
         durations.append((a, b))
-    torch.cuda.synchronize()
-    per_call_ms = sum(a.elapsed_time(b) for a, b in durations) / len(durations)
+    stream.synchronize()
+    per_call_ms = sum(a.elapsed_ms(b) for a, b in durations) / len(durations)
     launches = fl.launches_per_call(spl)
     per_launch_ms = per_call_ms / launches
     alg_bytes_per_launch = 17 * spl * W * H / launches

@@ -272,6 +272,26 @@ int osy_batch_get_tuning(const osy_batch* batch, osy_batch_tuning* tuning);
 int osy_batch_autotune(osy_batch* batch, osy_batch_tuning* chosen, double* ms);
 int osy_batch_get_info(const osy_batch* batch, osy_batch_info* info);
 
+/* ---- streams and events of the library's HIP runtime (no counterpart in
+ * the reference).  For callers that order or time the solver's work without
+ * a second HIP runtime in the process (a framework that bundles its own
+ * libamdhip64 would otherwise hand the library foreign stream handles).
+ * Streams are non-blocking with the default priority. */
+int osy_stream_create(void** stream);
+int osy_stream_destroy(void* stream);
+int osy_stream_synchronize(void* stream);
+int osy_event_create(void** event);
+int osy_event_destroy(void* event);
+/* Record `event` on `stream` (NULL = the null stream). */
+int osy_event_record(void* event, void* stream);
+int osy_event_synchronize(void* event);
+/* Milliseconds between two recorded, completed events. */
+int osy_event_elapsed_ms(void* start, void* end, float* ms);
+/* `reps` device-to-device copies of `bytes` bytes (a multiple of 16) with
+ * the library's own copy kernel, between two fresh buffers; synchronous.
+ * For scaling hardware byte counters against a known traffic. */
+int osy_calibration_copy(uint64_t bytes, int32_t reps);
+
 /* Message of the calling thread's last failed call ("" if none). */
 const char* osy_last_error(void);
 /* OSY_ABI_VERSION of the loaded library. */

@@ -24,11 +24,76 @@ import numpy as np
 from ._abi import (FIELDS, STORAGE, OpenSayalError, OsyBatchInfo, OsyBatchTuning, OsyConfig,
                    OsyFluidInfo, OsySource, OsyTuning, check, lib)
 
-__all__ = ["Batch", "Config", "ConfigParser", "Fluid", "Source", "OpenSayalError",
-           "default_config", "device_bytes", "FIELDS", "STORAGE"]
+__all__ = ["Batch", "Config", "ConfigParser", "Event", "Fluid", "Source", "Stream",
+           "OpenSayalError", "default_config", "device_bytes", "FIELDS", "STORAGE"]
 
 Config = OsyConfig
 Source = OsySource
+
+
+def _stream_ptr(stream) -> ctypes.c_void_p:
+    """A ``Stream``, a raw hipStream_t value of the library's runtime, or
+    None (the handle's own stream)."""
+    if isinstance(stream, Stream):
+        return ctypes.c_void_p(stream.handle)
+    return ctypes.c_void_p(stream or None)
+
+
+class Stream:
+    """A HIP stream of the library's own runtime (non-blocking, default
+    priority).  ``handle`` is what the ``stream`` arguments below take."""
+
+    def __init__(self):
+        h = ctypes.c_void_p()
+        check(lib().osy_stream_create(ctypes.byref(h)))
+        self.handle = h.value
+
+    def synchronize(self) -> None:
+        check(lib().osy_stream_synchronize(ctypes.c_void_p(self.handle)))
+
+    def close(self) -> None:
+        if self.handle:
+            check(lib().osy_stream_destroy(ctypes.c_void_p(self.handle)))
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Event:
+    """A HIP timing event of the library's runtime."""
+
+    def __init__(self):
+        h = ctypes.c_void_p()
+        check(lib().osy_event_create(ctypes.byref(h)))
+        self.handle = h.value
+
+    def record(self, stream: "Stream | None" = None) -> None:
+        s = stream.handle if stream is not None else None
+        check(lib().osy_event_record(ctypes.c_void_p(self.handle), ctypes.c_void_p(s)))
+
+    def synchronize(self) -> None:
+        check(lib().osy_event_synchronize(ctypes.c_void_p(self.handle)))
+
+    def elapsed_ms(self, end: "Event") -> float:
+        ms = ctypes.c_float()
+        check(lib().osy_event_elapsed_ms(ctypes.c_void_p(self.handle),
+                                         ctypes.c_void_p(end.handle), ctypes.byref(ms)))
+        return ms.value
+
+    def close(self) -> None:
+        if self.handle:
+            check(lib().osy_event_destroy(ctypes.c_void_p(self.handle)))
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def default_config(width: int = 1920, height: int = 1080, **overrides) -> Config:
@@ -163,18 +228,18 @@ class Fluid:
         dt = self.config.d_t if d_t is None else d_t
         check(lib().osy_fluid_step(self._h, int(steps), ctypes.c_float(dt)))
 
-    def step_async(self, steps: int, d_t: Optional[float] = None, stream: int = 0) -> None:
+    def step_async(self, steps: int, d_t: Optional[float] = None, stream=None) -> None:
         dt = self.config.d_t if d_t is None else d_t
         check(lib().osy_fluid_step_async(self._h, int(steps), ctypes.c_float(dt),
-                                         ctypes.c_void_p(stream or None)))
+                                         _stream_ptr(stream)))
 
     def synchronize(self) -> None:
         check(lib().osy_fluid_synchronize(self._h))
 
-    def project(self, sweeps: Optional[int] = None, stream: int = 0, sync: bool = True) -> None:
+    def project(self, sweeps: Optional[int] = None, stream=None, sync: bool = True) -> None:
         """Fluid::apply_projection alone (``sweeps`` defaults to projection n)."""
         n = self.config.projection_n if sweeps is None else sweeps
-        check(lib().osy_fluid_project(self._h, int(n), ctypes.c_void_p(stream or None)))
+        check(lib().osy_fluid_project(self._h, int(n), _stream_ptr(stream)))
         if sync:
             self.synchronize()
 
@@ -245,8 +310,8 @@ class Batch:
     def step(self, steps: int = 1) -> None:
         check(lib().osy_batch_step(self._h, steps))
 
-    def step_async(self, steps: int = 1, stream: int = 0) -> None:
-        check(lib().osy_batch_step_async(self._h, steps, ctypes.c_void_p(stream or None)))
+    def step_async(self, steps: int = 1, stream=None) -> None:
+        check(lib().osy_batch_step_async(self._h, steps, _stream_ptr(stream)))
 
     def synchronize(self) -> None:
         check(lib().osy_batch_synchronize(self._h))

@@ -137,6 +137,16 @@ PROTOTYPES = {
     "osy_batch_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(OsyBatchTuning),
                                           ctypes.POINTER(ctypes.c_double)]),
     "osy_batch_get_info": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(OsyBatchInfo)]),
+    "osy_stream_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p)]),
+    "osy_stream_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "osy_stream_synchronize": (ctypes.c_int, [ctypes.c_void_p]),
+    "osy_event_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p)]),
+    "osy_event_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "osy_event_record": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "osy_event_synchronize": (ctypes.c_int, [ctypes.c_void_p]),
+    "osy_event_elapsed_ms": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p,
+                                            ctypes.POINTER(_f32)]),
+    "osy_calibration_copy": (ctypes.c_int, [ctypes.c_uint64, _i32]),
 }
 
 _lib = None

@@ -607,3 +607,75 @@ int osy_fluid_get_info(const osy_fluid* fl, osy_fluid_info* info) {
 }
 
 }  // extern "C"
+
+extern "C" {
+
+int osy_stream_create(void** stream) {
+  if (!stream) return set_error(OSY_E_INVALID, "null argument");
+  hipStream_t s = nullptr;
+  OSY_HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  *stream = s;
+  return OSY_OK;
+}
+
+int osy_stream_destroy(void* stream) {
+  if (stream) OSY_HIP_CHECK(hipStreamDestroy((hipStream_t)stream));
+  return OSY_OK;
+}
+
+int osy_stream_synchronize(void* stream) {
+  OSY_HIP_CHECK(hipStreamSynchronize((hipStream_t)stream));
+  return OSY_OK;
+}
+
+int osy_event_create(void** event) {
+  if (!event) return set_error(OSY_E_INVALID, "null argument");
+  hipEvent_t e = nullptr;
+  OSY_HIP_CHECK(hipEventCreate(&e));
+  *event = e;
+  return OSY_OK;
+}
+
+int osy_event_destroy(void* event) {
+  if (event) OSY_HIP_CHECK(hipEventDestroy((hipEvent_t)event));
+  return OSY_OK;
+}
+
+int osy_event_record(void* event, void* stream) {
+  if (!event) return set_error(OSY_E_INVALID, "null event");
+  OSY_HIP_CHECK(hipEventRecord((hipEvent_t)event, (hipStream_t)stream));
+  return OSY_OK;
+}
+
+int osy_event_synchronize(void* event) {
+  if (!event) return set_error(OSY_E_INVALID, "null event");
+  OSY_HIP_CHECK(hipEventSynchronize((hipEvent_t)event));
+  return OSY_OK;
+}
+
+int osy_event_elapsed_ms(void* start, void* end, float* ms) {
+  if (!start || !end || !ms) return set_error(OSY_E_INVALID, "null argument");
+  OSY_HIP_CHECK(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)end));
+  return OSY_OK;
+}
+
+int osy_calibration_copy(uint64_t bytes, int32_t reps) {
+  if (bytes == 0 || bytes % 16 != 0 || reps < 0)
+    return set_error(OSY_E_INVALID, "bytes must be a positive multiple of 16, reps >= 0");
+  void* a = nullptr;
+  void* b = nullptr;
+  hipError_t e = hipMalloc(&a, bytes);
+  if (e == hipSuccess) e = hipMalloc(&b, bytes);
+  if (e == hipSuccess) e = hipMemset(a, 0, bytes);
+  for (int32_t r = 0; r < reps && e == hipSuccess; r++) {
+    enqueue_copy(b, a, bytes, nullptr);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (a) (void)hipFree(a);
+  if (b) (void)hipFree(b);
+  if (e != hipSuccess) return set_error(OSY_E_HIP, std::string("copy: ") + hipGetErrorString(e));
+  return OSY_OK;
+}
+
+}  // extern "C"

@@ -139,4 +139,8 @@ void enqueue_export(int field, int storage, const StepParams& sp,
 void enqueue_import(int field, int storage, const StepParams& sp,
                     const FieldPtrs& f, int uv, int sm, const void* ref, hipStream_t s);
 
+// Plain device-to-device copy of `bytes` (a multiple of 16) with 16-byte
+// loads and stores, for calibrating hardware byte counters.
+void enqueue_copy(void* dst, const void* src, size_t bytes, hipStream_t s);
+
 }  // namespace osy

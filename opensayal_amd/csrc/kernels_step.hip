@@ -337,6 +337,22 @@ void enqueue_projection_only(int storage, const StepParams& sp_in, const FieldPt
   }
 }
 
+namespace {
+__global__ void __launch_bounds__(256) k_copy(uint4* __restrict__ dst,
+                                              const uint4* __restrict__ src, size_t n) {
+  for (size_t k = (size_t)blockIdx.x * 256 + threadIdx.x; k < n; k += (size_t)gridDim.x * 256)
+    dst[k] = src[k];
+}
+}  // namespace
+
+void enqueue_copy(void* dst, const void* src, size_t bytes, hipStream_t s) {
+  const size_t n = bytes / 16;
+  size_t blocks = (n + 255) / 256;
+  if (blocks > 8192) blocks = 8192;
+  hipLaunchKernelGGL(k_copy, dim3((unsigned)blocks), dim3(256), 0, s, (uint4*)dst,
+                     (const uint4*)src, n);
+}
+
 void enqueue_pressure_range(const StepParams& sp, const FieldPtrs& f, hipStream_t s) {
   hipLaunchKernelGGL(k_minmax_partial, dim3(kReduceBlocks), dim3(256), 0, s, sp, f.p, f.scratch);
   hipLaunchKernelGGL(k_minmax_final, dim3(1), dim3(256), 0, s, f.scratch, kReduceBlocks, f.prange);

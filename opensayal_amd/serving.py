@@ -92,16 +92,14 @@ class DeviceBackend:
     """The GPU side: one ``Batch`` whose members are the resident requests."""
 
     def __init__(self, tuning: Optional[dict] = None, collect: Sequence[str] = ()):
-        import torch
-
-        from . import Batch
+        from . import Batch, Event, Stream
 
         self.batch = Batch()
         if tuning:
             self.batch.set_tuning(**tuning)
         self.collect = tuple(collect)
-        self.stream = torch.cuda.Stream()
-        self._torch = torch
+        self.stream = Stream()
+        self._events = (Event(), Event())
 
     def bytes(self, spec: SimSpec) -> int:
         from . import device_bytes
@@ -117,14 +115,12 @@ class DeviceBackend:
 
     def step(self, steps: int) -> float:
         """Runs `steps` batch steps; returns the GPU milliseconds they took."""
-        t = self._torch
-        a, b = t.cuda.Event(enable_timing=True), t.cuda.Event(enable_timing=True)
+        a, b = self._events
         a.record(self.stream)
-        self.batch.step_async(steps, stream=self.stream.cuda_stream)
+        self.batch.step_async(steps, stream=self.stream)
         b.record(self.stream)
-        b.synchronize()
         self.batch.synchronize()
-        return a.elapsed_time(b)
+        return a.elapsed_ms(b)
 
     def retire(self, handle) -> Optional[Dict[str, np.ndarray]]:
         out = {n: handle.get_field(n) for n in self.collect} if self.collect else None
@@ -134,6 +130,7 @@ class DeviceBackend:
 
     def close(self) -> None:
         self.batch.close()
+        self.stream.close()
 
 
 class Server:

@@ -276,3 +276,24 @@ def test_errors_are_reported():
         fl.set_field("is_solid", np.zeros((36, 64), np.float32))
     with pytest.raises(ValueError):
         fl.set_field("vel_x", np.zeros((10, 10), np.float32))
+
+
+def test_library_stream_and_events():
+    """Steps enqueued on a library Stream give the same fields as synchronous
+    steps, and events recorded around them time a positive interval."""
+    from opensayal_amd import Event, Stream
+
+    cfg = wl.plain_small()
+    a, _ = pair(cfg, seed=15)
+    b, _ = pair(cfg, seed=15)
+    s = Stream()
+    e0, e1 = Event(), Event()
+    e0.record(s)
+    a.step_async(4, stream=s)
+    e1.record(s)
+    s.synchronize()
+    assert e0.elapsed_ms(e1) > 0
+    b.step(4)
+    for n, _ in FIELDS:
+        assert np.array_equal(a.get_field(n), b.get_field(n))
+    s.close()
