@@ -193,6 +193,8 @@ def main() -> int:
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only device-to-device copies of a known size (to scale "
                          "FETCH_SIZE / WRITE_SIZE counters)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default projection shape (for profiling runs)")
     ap.add_argument("--sweep", action="store_true",
                     help="time a grid of projection launch shapes instead")
     args = ap.parse_args()
@@ -215,7 +217,7 @@ def main() -> int:
 
     if args.workload in ("batched", "mixed_small"):
         specs = wl.batched_specs() if args.workload == "batched" else wl.small_specs()
-        out = batch_leg(specs, args.steps, args.warmup)
+        out = batch_leg(specs, args.steps, args.warmup, autotune=not args.no_autotune)
         out.update({"metric": METRIC, "n_gpus": 1, "higher_is_better": True,
                     "scaling": "weak", "vs_baseline": None, "dtype": "f32",
                     "data": "synthetic: seeded +-0.01 velocity noise, smoke 1 on the inflow",
@@ -240,7 +242,10 @@ def main() -> int:
         return 0
 
     # the library's autotuner picks the projection launch shape (not timed)
-    tuning, tune_ms = fl.autotune()
+    if args.no_autotune:
+        tuning, tune_ms = fl.tuning, float("nan")
+    else:
+        tuning, tune_ms = fl.autotune()
 
     fl.step_async(args.warmup, stream=stream)
     stream.synchronize()

@@ -27,21 +27,25 @@ union BatchShared {
   AdvShared<kBatchAdvTY> adv;
 };
 
-// Thread 0: wait until *counter >= target.  Returns false on timeout, or
-// at once when an earlier wait of the launch timed out (so a failed launch
-// drains in about one bound rather than one per dependency level).
+// Thread 0: wait until *counter >= target, polling with relaxed loads (an
+// acquire per poll would invalidate the L2 each time), then one agent-scope
+// acquire.  Returns false on timeout, or at once when an earlier wait of the
+// launch timed out (so a failed launch drains in about one bound rather than
+// one per dependency level).
 __device__ inline bool wait_count(const unsigned long long* counter, unsigned long long target,
                                   const int* error, long long spin_ticks) {
-  if (__hip_atomic_load(counter, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) >= target)
-    return true;
-  const long long t0 = wall_clock64();
-  for (;;) {
-    __builtin_amdgcn_s_sleep(2);
-    if (__hip_atomic_load(counter, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) >= target)
-      return true;
-    if (__hip_atomic_load(error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return false;
-    if (wall_clock64() - t0 > spin_ticks) return false;
+  if (__hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+    const long long t0 = wall_clock64();
+    for (;;) {
+      __builtin_amdgcn_s_sleep(2);
+      if (__hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= target)
+        break;
+      if (__hip_atomic_load(error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return false;
+      if (wall_clock64() - t0 > spin_ticks) return false;
+    }
   }
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  return true;
 }
 
 template <typename T, bool PRESSURE, int R>
@@ -127,15 +131,17 @@ __global__ void __launch_bounds__(kBatchThreads) k_batch_step(BatchArgs A) {
       if (!ok) __hip_atomic_store(A.error, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       s_ok = ok ? 1 : 0;
     }
+    // thread 0's acquire invalidated the caches this workgroup reads
+    // through (its CU's L1 and its XCD's L2); the barrier orders every
+    // wavefront's loads after it
     __syncthreads();
-    // every wavefront orders its loads after thread 0's acquire
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     if (__builtin_amdgcn_readfirstlane(s_ok)) {
       const int simi = __builtin_amdgcn_readfirstlane(st.sim);
       run_item<T, PRESSURE, R>(A, st, A.sims[simi], tile, (unsigned)(epoch & 1ULL), sh);
     }
-    // every wavefront's stores are visible at agent scope before the count
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    // every wavefront's stores have reached the L2 before the barrier; thread
+    // 0's release then writes the L2 back once for the whole workgroup
+    __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     if (threadIdx.x == 0) {
       __hip_atomic_fetch_add(A.done + sid, 1ULL, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
