@@ -33,7 +33,18 @@ struct AdvRegion {
            (unsigned)(j - y0) < (unsigned)AdvShared<TY>::RY;
   }
   __device__ int at(int i, int j) const { return (j - y0) * kAdvRX + (i - x0); }
+  // every cell within one of (i, j) is staged
+  __device__ bool inner(int i, int j) const {
+    return (unsigned)(i - x0 - 1) < (unsigned)(kAdvRX - 2) &&
+           (unsigned)(j - y0 - 1) < (unsigned)(AdvShared<TY>::RY - 2);
+  }
 };
+
+// Cell of a sample position (the index computation the samplers start with).
+template <bool CS1>
+__device__ inline int sample_cell(float x, int cs) {
+  return to_index(div_cs<CS1>(x, cs));
+}
 
 // ---- P1: apply_external_forces_at (src/fluid.cu:308-349), zero_pressure,
 // over the tile at (bx, by).  Only the parts no later phase applies while
@@ -97,6 +108,10 @@ __device__ inline void advect_velocity_tile(const StepParams& sp, const GridView
   auto vval = [&](int i, int j) -> float {
     return R.in(i, j) ? sv[R.at(i, j)] : v_extrapolated(v, g, i, j);
   };
+  // staged-only accessors (no bounds test) for cells known to be staged
+  auto svalid = [&](int i, int j) -> bool { return (sf[R.at(i, j)] & CF_FLUID) != 0; };
+  auto suval = [&](int i, int j) -> float { return su[R.at(i, j)]; };
+  auto svval = [&](int i, int j) -> float { return sv[R.at(i, j)]; };
   const int cs = sp.cs;
   const float dt = sp.d_t;
   const int i = bx * kAdvTX + (tid & 63);
@@ -105,34 +120,45 @@ __device__ inline void advect_velocity_tile(const StepParams& sp, const GridView
     const int j = by * TY + (tid >> 6) + NY * m;
     if (i >= sp.W || j >= sp.H) continue;
     const int l = R.at(i, j);
-    float nu, nv;
-    {  // u face at (i*cs, (j+0.5)*cs); get_vertical_edge_velocity (:364-389)
-      const float px = (float)(CS1 ? i : i * cs);
-      const float py = CS1 ? (float)j + 0.5f : (float)(((double)j + 0.5) * (double)cs);
-      const float vel_x = su[l];
-      float avg_v = sv[l];
-      int count = 1;
-      if (valid(i - 1, j + 1)) { avg_v = avg_v + vval(i - 1, j + 1); count++; }
-      if (valid(i, j + 1)) { avg_v = avg_v + vval(i, j + 1); count++; }
-      if (valid(i - 1, j)) { avg_v = avg_v + vval(i - 1, j); count++; }
-      avg_v = div_count(avg_v, count);
-      const float x = px - vel_x * dt;
-      const float y = py - avg_v * dt;
-      nu = sample_u<CS1>(cs, x, y, valid, uval);
+    // the edge velocities read the 3x3 neighbourhood of a tile cell, which
+    // is staged; the same terms in the same order as
+    // get_vertical_edge_velocity (:364-389) and
+    // get_horizontal_edge_velocity (:391-416), skipped terms adding -0
+    // u face at (i*cs, (j+0.5)*cs)
+    const float pux = (float)(CS1 ? i : i * cs);
+    const float puy = CS1 ? (float)j + 0.5f : (float)(((double)j + 0.5) * (double)cs);
+    float avg_v = sv[l];
+    {
+      const bool a = svalid(i - 1, j + 1), b = svalid(i, j + 1), c = svalid(i - 1, j);
+      avg_v = add_if(avg_v, a, svval(i - 1, j + 1));
+      avg_v = add_if(avg_v, b, svval(i, j + 1));
+      avg_v = add_if(avg_v, c, svval(i - 1, j));
+      avg_v = div_count_bf(avg_v, 1 + (int)a + (int)b + (int)c);
     }
-    {  // v face at ((i+0.5)*cs, j*cs); get_horizontal_edge_velocity (:391-416)
-      const float px = CS1 ? (float)i + 0.5f : (float)(((double)i + 0.5) * (double)cs);
-      const float py = (float)(CS1 ? j : j * cs);
-      const float vel_y = sv[l];
-      float avg_u = su[l];
-      int count = 1;
-      if (valid(i + 1, j)) { avg_u = avg_u + uval(i + 1, j); count++; }
-      if (valid(i, j - 1)) { avg_u = avg_u + uval(i, j - 1); count++; }
-      if (valid(i + 1, j - 1)) { avg_u = avg_u + uval(i + 1, j - 1); count++; }
-      avg_u = div_count(avg_u, count);
-      const float x = px - avg_u * dt;
-      const float y = py - vel_y * dt;
-      nv = sample_v<CS1>(cs, x, y, valid, vval);
+    const float xu = pux - su[l] * dt;
+    const float yu = puy - avg_v * dt;
+    // v face at ((i+0.5)*cs, j*cs)
+    const float pvx = CS1 ? (float)i + 0.5f : (float)(((double)i + 0.5) * (double)cs);
+    const float pvy = (float)(CS1 ? j : j * cs);
+    float avg_u = su[l];
+    {
+      const bool a = svalid(i + 1, j), b = svalid(i, j - 1), c = svalid(i + 1, j - 1);
+      avg_u = add_if(avg_u, a, suval(i + 1, j));
+      avg_u = add_if(avg_u, b, suval(i, j - 1));
+      avg_u = add_if(avg_u, c, suval(i + 1, j - 1));
+      avg_u = div_count_bf(avg_u, 1 + (int)a + (int)b + (int)c);
+    }
+    const float xv = pvx - avg_u * dt;
+    const float yv = pvy - sv[l] * dt;
+    float nu, nv;
+    const bool staged = R.inner(sample_cell<CS1>(xu, cs), sample_cell<CS1>(yu, cs)) &&
+                        R.inner(sample_cell<CS1>(xv, cs), sample_cell<CS1>(yv, cs));
+    if (__all(staged)) {
+      nu = sample_u_bf<CS1>(cs, xu, yu, svalid, suval);
+      nv = sample_v_bf<CS1>(cs, xv, yv, svalid, svval);
+    } else {
+      nu = sample_u<CS1>(cs, xu, yu, valid, uval);
+      nv = sample_v<CS1>(cs, xv, yv, valid, vval);
     }
     stf(u_out, g.at(i, j), nu);
     stf(v_out, g.at(i, j), nv);
@@ -176,6 +202,10 @@ __device__ inline void advect_smoke_tile(const StepParams& sp, const GridView& g
   auto sval = [&](int i, int j) -> float {
     return R.in(i, j) ? ss[R.at(i, j)] : force_smoke(sp, i, j, ldf(smoke, g.at(i, j)));
   };
+  auto svalid = [&](int i, int j) -> bool { return (sf[R.at(i, j)] & CF_FLUID) != 0; };
+  auto suval = [&](int i, int j) -> float { return su[R.at(i, j)]; };
+  auto svval = [&](int i, int j) -> float { return sv[R.at(i, j)]; };
+  auto ssval = [&](int i, int j) -> float { return ss[R.at(i, j)]; };
   const int cs = sp.cs;
   const int i = bx * kAdvTX + (tid & 63);
 #pragma unroll
@@ -183,11 +213,17 @@ __device__ inline void advect_smoke_tile(const StepParams& sp, const GridView& g
     const int j = by * TY + (tid >> 6) + NY * m;
     if (i >= sp.W || j >= sp.H) continue;
     const float cx = centre<CS1>(i, cs), cy = centre<CS1>(j, cs);
-    const float vx = sample_u<CS1>(cs, cx, cy, valid, uval);
-    const float vy = sample_v<CS1>(cs, cx, cy, valid, vval);
+    // the velocity at a cell centre samples cells within one of (i, j),
+    // which are staged
+    const float vx = sample_u_bf<CS1>(cs, cx, cy, svalid, suval);
+    const float vy = sample_v_bf<CS1>(cs, cx, cy, svalid, svval);
     const float x = cx - vx * sp.d_t;
     const float y = cy - vy * sp.d_t;
-    float s = sample_smoke<CS1>(cs, x, y, valid, sval);
+    float s;
+    if (__all(R.inner(sample_cell<CS1>(x, cs), sample_cell<CS1>(y, cs))))
+      s = sample_smoke_bf<CS1>(cs, x, y, svalid, ssval);
+    else
+      s = sample_smoke<CS1>(cs, x, y, valid, sval);
     if (sp.decay_on) s = (float)fmax((double)(s - sp.decay_rd), 0.0);
     stf(smoke_out, g.at(i, j), s);
   }

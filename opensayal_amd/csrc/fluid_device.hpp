@@ -98,6 +98,26 @@ __device__ inline float div_count(float x, int n) {
   return x * __uint_as_float(0x3F800000u - ((uint32_t)(n >> 1) << 23));
 }
 
+// x / 3 exactly: q0 = RN(x * RN(1/3)) corrected once with fused multiply-adds
+// (q0 itself when it is zero or infinite).  Equal to IEEE x / 3.0f for every
+// binary32 x (exhaustive check in tests/csrc/quotient_check.c).
+__device__ inline float third_quotient(float x) {
+  const float y = 1.0f / 3.0f;
+  const float q0 = x * y;
+  const float q1 = __builtin_fmaf(__builtin_fmaf(-q0, 3.0f, x), y, q0);
+  return (q0 == 0.0f || __builtin_isinf(q0)) ? q0 : q1;
+}
+
+// div_count without a branch: the n = 3 quotient through third_quotient.
+__device__ inline float div_count_bf(float x, int n) {
+  const float p2 = x * __uint_as_float(0x3F800000u - ((uint32_t)(n >> 1) << 23));
+  return n == 3 ? third_quotient(x) : p2;
+}
+
+// acc + t when `on`, else acc + -0 (which is acc, bit for bit, for every
+// acc): the branch-free form of `if (on) acc = acc + t`.
+__device__ inline float add_if(float acc, bool on, float t) { return acc + (on ? t : -0.0f); }
+
 // get_general_velocity_x (src/fluid.cu:479-539).  valid(i, j) is
 // Fluid::is_valid_fluid, val(i, j) reads u.
 template <bool CS1, class Valid, class Val>
@@ -161,6 +181,61 @@ __device__ inline float sample_v(int cs, float x, float y, const Valid& valid, c
   return acc;
 }
 
+// sample_u, sample_v and sample_smoke without branches, for accessors that
+// may be called for any of the sampled cells (staged cells in LDS).  The
+// terms are added in the same order as the branching forms, skipped terms
+// become + -0, and the two orders of the four terms are selected, so the
+// results are the same bits.
+template <bool CS1, class Valid, class Val>
+__device__ inline float sample_u_bf(int cs, float x, float y, const Valid& valid, const Val& val) {
+  const int i = to_index(div_cs<CS1>(x, cs));
+  const int j = to_index(div_cs<CS1>(y, cs));
+  const float in_x = x - (float)(CS1 ? i : i * cs);
+  const float in_y = y - (float)(CS1 ? j : j * cs);
+  const double h = cs / 2.0;
+  const bool low = CS1 ? (in_y <= 0.5f) : ((double)in_y <= h);
+  const float d_lo = CS1 ? 0.5f - in_y : (float)(h - (double)in_y);
+  const float d_hi = CS1 ? in_y - 0.5f : (float)((double)in_y - h);
+  const float w_x = 1.0f - div_cs<CS1>(in_x, cs);
+  const float w_y = 1.0f - div_cs<CS1>(low ? d_lo : d_hi, cs);
+  const int jo = low ? j - 1 : j + 1;
+  const float tA = (w_y * w_x) * val(i, j);
+  const float tB = (w_y * (1.0f - w_x)) * val(i + 1, j);
+  const float tC = ((1.0f - w_y) * w_x) * val(i, jo);
+  const float tD = ((1.0f - w_y) * (1.0f - w_x)) * val(i + 1, jo);
+  const bool vB = valid(i + 1, j), vC = valid(i, jo), vD = valid(i + 1, jo);
+  float acc = 0.0f + tA;
+  acc = low ? add_if(acc, vB, tB) : add_if(acc, vC, tC);
+  acc = low ? add_if(acc, vC, tC) : add_if(acc, vB, tB);
+  acc = add_if(acc, vD, tD);
+  return valid(i, j) ? acc : 0.0f;
+}
+
+template <bool CS1, class Valid, class Val>
+__device__ inline float sample_v_bf(int cs, float x, float y, const Valid& valid, const Val& val) {
+  const int i = to_index(div_cs<CS1>(x, cs));
+  const int j = to_index(div_cs<CS1>(y, cs));
+  const float in_x = x - (float)(CS1 ? i : i * cs);
+  const float in_y = y - (float)(CS1 ? j : j * cs);
+  const double h = cs / 2.0;
+  const bool left = CS1 ? (in_x < 0.5f) : ((double)in_x < h);
+  const float d_l = CS1 ? 0.5f - in_x : (float)(h - (double)in_x);
+  const float d_r = CS1 ? in_x - 0.5f : (float)((double)in_x - h);
+  const float w_x = 1.0f - div_cs<CS1>(left ? d_l : d_r, cs);
+  const float w_y = 1.0f - div_cs<CS1>(in_y, cs);
+  const int io = left ? i - 1 : i + 1;
+  const float tA = (w_y * w_x) * val(i, j);
+  const float tB = (w_y * (1.0f - w_x)) * val(io, j);
+  const float tC = ((1.0f - w_y) * (1.0f - w_x)) * val(io, j + 1);
+  const float tD = ((1.0f - w_y) * w_x) * val(i, j + 1);
+  const bool vB = valid(io, j), vC = valid(io, j + 1), vD = valid(i, j + 1);
+  float acc = 0.0f + tA;
+  acc = left ? add_if(acc, vB, tB) : add_if(acc, vD, tD);
+  acc = add_if(acc, vC, tC);
+  acc = left ? add_if(acc, vD, tD) : add_if(acc, vB, tB);
+  return valid(i, j) ? acc : 0.0f;
+}
+
 // get_center_position (src/fluid.cu:547-550): ((a + 0.5) * cs) in double,
 // rounded to float.  With cs == 1, (float)a + 0.5f rounds the same exact value.
 template <bool CS1> __device__ inline float centre(int a, int cs) {
@@ -200,6 +275,33 @@ __device__ inline float sample_smoke(int cs, float x, float y, const Valid& vali
   if (valid(i2, j)) acc = acc + (inv2 / sum_inv) * val(i2, j);
   if (valid(i, j3)) acc = acc + (inv3 / sum_inv) * val(i, j3);
   if (valid(i2, j3)) acc = acc + (inv4 / sum_inv) * val(i2, j3);
+  return acc;
+}
+
+template <bool CS1, class Valid, class Val>
+__device__ inline float sample_smoke_bf(int cs, float x, float y, const Valid& valid,
+                                        const Val& val) {
+  const int i = to_index(div_cs<CS1>(x, cs));
+  const int j = to_index(div_cs<CS1>(y, cs));
+  const float in_x = x - (float)(CS1 ? i : i * cs);
+  const float in_y = y - (float)(CS1 ? j : j * cs);
+  const double h = cs / 2.0;
+  const bool left = CS1 ? (in_x < 0.5f) : ((double)in_x < h);
+  const bool low = CS1 ? (in_y < 0.5f) : ((double)in_y < h);
+  const int i2 = left ? i - 1 : i + 1;
+  const int j3 = low ? j - 1 : j + 1;
+  const float x1 = centre<CS1>(i, cs), y1 = centre<CS1>(j, cs);
+  const float x2 = centre<CS1>(i2, cs), y3 = centre<CS1>(j3, cs);
+  const float inv1 = inv_distance(x - x1, y - y1);
+  const float inv2 = inv_distance(x - x2, y - y1);
+  const float inv3 = inv_distance(x - x1, y - y3);
+  const float inv4 = inv_distance(x - x2, y - y3);
+  const float sum_inv = ((inv1 + inv2) + inv3) + inv4;
+  float acc = 0.0f;
+  acc = add_if(acc, valid(i, j), (inv1 / sum_inv) * val(i, j));
+  acc = add_if(acc, valid(i2, j), (inv2 / sum_inv) * val(i2, j));
+  acc = add_if(acc, valid(i, j3), (inv3 / sum_inv) * val(i, j3));
+  acc = add_if(acc, valid(i2, j3), (inv4 / sum_inv) * val(i2, j3));
   return acc;
 }
 

@@ -35,6 +35,19 @@ __device__ inline float relaxed_quotient(float div, uint32_t g, float o) {
   return o * q;
 }
 
+// The same quotient for the register-resident kernel: `slow` (wavefront
+// uniform) says whether any lane of this row may need the division; only
+// such rows divide, every lane, and select.
+__device__ inline float row_quotient(float div, uint32_t g, float o, bool slow) {
+  const float y = __uint_as_float(0x3F800000u - (((g >> G_L2_SHIFT) & 3u) << 23));
+  float q = div * y;
+  if (slow) {
+    const float qs = div / (float)__builtin_popcount(g & 15u);
+    q = (g & G_SLOW) ? qs : q;
+  }
+  return o * q;
+}
+
 // x + gate(d): d when the gate bit is set, else -0 (x + -0 == x for every x,
 // signed zeros and NaN included), so the face keeps its bits when closed.
 __device__ inline float gated_add(float x, float d, uint32_t g, int bit) {
@@ -147,6 +160,8 @@ struct RegTile {
   float u0[R], u1[R], v0[R], v1[R];  // faces of cells 2l and 2l+1, rows 0..R-1
   float s0[R], s1[R];                // pressure sums (pressure runs only)
   uint32_t gw[R];                    // gate bytes: cell 2l in bits 0-7, 2l+1 in 8-15
+  uint32_t slow;                     // wavefront-uniform: bit 2 rho + c set when some
+                                     // lane's cell c of row rho divides (total_s 0 or 3)
 };
 
 // apply_projection_at for the active cell of row rho (parity B) of every
@@ -162,7 +177,7 @@ __device__ inline void reg_row(RegTile<R>& t, const int rho, uint32_t mr, uint32
     const float uu = t.u0[rho], ur = t.u1[rho], vv = t.v0[rho];
     const float vt = (rho + 1 < R) ? t.v0[up] : vt0;
     const float div = ((ur - uu) + vt) - vv;
-    const float d = relaxed_quotient(div, g, o);
+    const float d = row_quotient(div, g, o, (t.slow >> (2 * rho)) & 1u);
     if constexpr (PRESSURE) {
       if (g & G_UPD) t.s0[rho] = t.s0[rho] + pressure_term(d, density, csf, d_t);
     }
@@ -179,7 +194,7 @@ __device__ inline void reg_row(RegTile<R>& t, const int rho, uint32_t mr, uint32
     const float uu = t.u1[rho], ur = lane_from_right(t.u0[rho]), vv = t.v1[rho];
     const float vt = (rho + 1 < R) ? t.v1[up] : vt1;
     const float div = ((ur - uu) + vt) - vv;
-    const float d = relaxed_quotient(div, g, o);
+    const float d = row_quotient(div, g, o, (t.slow >> (2 * rho + 1)) & 1u);
     if constexpr (PRESSURE) {
       if (g & G_UPD) t.s1[rho] = t.s1[rho] + pressure_term(d, density, csf, d_t);
     }
@@ -278,6 +293,7 @@ __device__ inline void project_regs_tile(const StepParams& sp, const uint8_t* __
   const bool col0 = (unsigned)gi < (unsigned)W, col1 = (unsigned)(gi + 1) < (unsigned)W;
 
   RegTile<R> t;
+  t.slow = 0;
   // flags: rows rbase .. rbase+R (one above for the top faces), columns
   // 2l, 2l+1 of this lane and 2l+2 (lane l+1's first column; lane 63 loads
   // its own)
@@ -312,6 +328,8 @@ __device__ inline void project_regs_tile(const StepParams& sp, const uint8_t* __
     }
     t.gw[rho] = gate_byte(fl0[rho], fl1[rho], fl0[rho + 1]) |
                 (gate_byte(fl1[rho], fl2[rho], fl1[rho + 1]) << 8);
+    t.slow |= (__any((t.gw[rho] & G_SLOW) != 0) ? 1u : 0u) << (2 * rho);
+    t.slow |= (__any((t.gw[rho] & (G_SLOW << 8)) != 0) ? 1u : 0u) << (2 * rho + 1);
   }
 
   const float o = sp.o;
