@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define OSY_ABI_VERSION 2
+#define OSY_ABI_VERSION 3
 
 typedef enum osy_status {
   OSY_OK = 0,
@@ -141,11 +141,13 @@ typedef struct osy_tuning {
 typedef struct osy_fluid osy_fluid;
 
 /* Projection launch shape of a batch: register-tile rows per wavefront (4 or
- * 8; the tile is (128 - 4k) x (8 * rows - 4k) cells with a 2k-cell halo) and
- * sweeps per tile k. */
+ * 8; the tile is (128 - 4k) x (8 * rows - 4k) cells with a 2k-cell halo),
+ * sweeps per tile k, and occupancy: 1 = the compiler's register allocation,
+ * 2 = registers capped for two workgroups per CU (rows 8 only); 0 means 1. */
 typedef struct osy_batch_tuning {
   int32_t rows;
   int32_t sweeps_per_tile;
+  int32_t occupancy;
 } osy_batch_tuning;
 
 typedef struct osy_batch_info {

@@ -320,10 +320,10 @@ class Batch:
     def tuning(self) -> dict:
         t = OsyBatchTuning()
         check(lib().osy_batch_get_tuning(self._h, ctypes.byref(t)))
-        return {"rows": t.rows, "sweeps_per_tile": t.sweeps_per_tile}
+        return {"rows": t.rows, "sweeps_per_tile": t.sweeps_per_tile, "occupancy": t.occupancy}
 
-    def set_tuning(self, rows: int = 8, sweeps_per_tile: int = 5) -> None:
-        t = OsyBatchTuning(rows, sweeps_per_tile)
+    def set_tuning(self, rows: int = 8, sweeps_per_tile: int = 5, occupancy: int = 1) -> None:
+        t = OsyBatchTuning(rows, sweeps_per_tile, occupancy)
         check(lib().osy_batch_set_tuning(self._h, ctypes.byref(t)))
 
     def autotune(self) -> tuple:
@@ -332,7 +332,8 @@ class Batch:
         t = OsyBatchTuning()
         ms = ctypes.c_double()
         check(lib().osy_batch_autotune(self._h, ctypes.byref(t), ctypes.byref(ms)))
-        return {"rows": t.rows, "sweeps_per_tile": t.sweeps_per_tile}, ms.value
+        return {"rows": t.rows, "sweeps_per_tile": t.sweeps_per_tile,
+                "occupancy": t.occupancy}, ms.value
 
     @property
     def info(self) -> dict:

@@ -89,7 +89,7 @@ class OsyFluidInfo(ctypes.Structure):
 class OsyBatchTuning(ctypes.Structure):
     """struct osy_batch_tuning: register-tile rows per wavefront, sweeps per tile."""
 
-    _fields_ = [("rows", _i32), ("sweeps_per_tile", _i32)]
+    _fields_ = [("rows", _i32), ("sweeps_per_tile", _i32), ("occupancy", _i32)]
 
 
 class OsyBatchInfo(ctypes.Structure):

@@ -51,7 +51,7 @@ __device__ inline int sample_cell(float x, int cs) {
 // loading (do_uv: velocity part and pressure zeroing; do_smoke: inflow
 // smoke).
 template <typename T, int TY, int NY>
-__device__ inline void forces_tile(const StepParams& sp, int do_uv, int do_smoke, T* u, T* v,
+__device__ __attribute__((always_inline)) inline void forces_tile(const StepParams& sp, int do_uv, int do_smoke, T* u, T* v,
                                    T* smoke, float* p, int bx, int by) {
   const int i = bx * kAdvTX + (int)(threadIdx.x & 63);
   if (i >= sp.W) return;
@@ -80,7 +80,7 @@ __device__ inline void forces_tile(const StepParams& sp, int do_uv, int do_smoke
 // are the extrapolated values (u_extrapolated / v_extrapolated), which are
 // also what the HBM fallback reads.  Ends with the LDS free for reuse.
 template <typename T, bool CS1, int TY, int NY>
-__device__ inline void advect_velocity_tile(const StepParams& sp, const GridView& g,
+__device__ __attribute__((always_inline)) inline void advect_velocity_tile(const StepParams& sp, const GridView& g,
                                             const T* __restrict__ u, const T* __restrict__ v,
                                             T* __restrict__ u_out, T* __restrict__ v_out, int bx,
                                             int by, AdvShared<TY>& sh) {
@@ -172,7 +172,7 @@ __device__ inline void advect_velocity_tile(const StepParams& sp, const GridView
 // the advected value before it is stored, which is the reference's
 // copy-back-then-decay.  Ends with the LDS free for reuse.
 template <typename T, bool CS1, int TY, int NY>
-__device__ inline void advect_smoke_tile(const StepParams& sp, const GridView& g,
+__device__ __attribute__((always_inline)) inline void advect_smoke_tile(const StepParams& sp, const GridView& g,
                                          const T* __restrict__ u, const T* __restrict__ v,
                                          const T* __restrict__ smoke, T* __restrict__ smoke_out,
                                          int bx, int by, AdvShared<TY>& sh) {
@@ -190,7 +190,7 @@ __device__ inline void advect_smoke_tile(const StepParams& sp, const GridView& g
     const long k = in ? g.at(i, j) : 0;
     su[q] = in ? ldf(u, k) : 0.0f;
     sv[q] = in ? ldf(v, k) : 0.0f;
-    ss[q] = in ? force_smoke(sp, i, j, ldf(smoke, k)) : 0.0f;
+    ss[q] = in ? rnd<T>(force_smoke(sp, i, j, ldf(smoke, k))) : 0.0f;
     sf[q] = in ? g.flags[k] : (uint8_t)0;
   }
   __syncthreads();
@@ -200,7 +200,7 @@ __device__ inline void advect_smoke_tile(const StepParams& sp, const GridView& g
   auto uval = [&](int i, int j) -> float { return R.in(i, j) ? su[R.at(i, j)] : ldf(u, g.at(i, j)); };
   auto vval = [&](int i, int j) -> float { return R.in(i, j) ? sv[R.at(i, j)] : ldf(v, g.at(i, j)); };
   auto sval = [&](int i, int j) -> float {
-    return R.in(i, j) ? ss[R.at(i, j)] : force_smoke(sp, i, j, ldf(smoke, g.at(i, j)));
+    return R.in(i, j) ? ss[R.at(i, j)] : rnd<T>(force_smoke(sp, i, j, ldf(smoke, g.at(i, j))));
   };
   auto svalid = [&](int i, int j) -> bool { return (sf[R.at(i, j)] & CF_FLUID) != 0; };
   auto suval = [&](int i, int j) -> float { return su[R.at(i, j)]; };
@@ -224,6 +224,7 @@ __device__ inline void advect_smoke_tile(const StepParams& sp, const GridView& g
       s = sample_smoke_bf<CS1>(cs, x, y, svalid, ssval);
     else
       s = sample_smoke<CS1>(cs, x, y, valid, sval);
+    s = rnd<T>(s);  // the advection's store, before the decay reads it back
     if (sp.decay_on) s = (float)fmax((double)(s - sp.decay_rd), 0.0);
     stf(smoke_out, g.at(i, j), s);
   }

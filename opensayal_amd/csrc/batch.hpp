@@ -67,15 +67,21 @@ struct BatchArgs {
   int tw, th, k;             // register-tile projection shape
 };
 
-// Register-tile rows per wavefront supported by the batch kernel.
+// Register-tile rows per wavefront supported by the batch kernel, and its
+// occupancy variants: 1 = registers as the compiler allocates them, 2 = capped
+// for two workgroups per CU (rows 8 only; rows 4 has two anyway).
 inline bool batch_rows_supported(int rows) { return rows == 4 || rows == 8; }
+inline bool batch_occupancy_supported(int rows, int occupancy) {
+  return occupancy == 1 || (occupancy == 2 && rows == 8);
+}
 
 // Occupancy-derived workgroup count for the batch kernel of this storage /
 // pressure / rows combination.
-hipError_t batch_grid_size(int storage, bool pressure, int rows, int* grid);
+hipError_t batch_grid_size(int storage, bool pressure, int rows, int occupancy, int* grid);
 
 // Enqueue one batched step (one launch).
-void enqueue_batch_step(int storage, bool pressure, int rows, const BatchArgs& a, hipStream_t s);
+void enqueue_batch_step(int storage, bool pressure, int rows, int occupancy, const BatchArgs& a,
+                        hipStream_t s);
 
 // [min, max] of the pressure into f.prange (thrust::reduce in
 // src/fluid.cu:778-787), for simulations stepped by a batch.

@@ -20,7 +20,7 @@ struct osy_batch {
   hipStream_t stream = nullptr;
   hipStream_t last = nullptr;  // stream of the last enqueued steps
   std::vector<osy_fluid*> members;
-  int rows = 8, k = 5;
+  int rows = 8, k = 5, occupancy = 1;
   bool dirty = true;
 
   struct Group {
@@ -222,7 +222,7 @@ int rebuild(osy_batch* b) {
     a.tw = tw;
     a.th = th;
     a.k = k;
-    OSY_HIP_CHECK(batch_grid_size(g.storage, g.pressure, rows, &a.grid));
+    OSY_HIP_CHECK(batch_grid_size(g.storage, g.pressure, rows, b->occupancy, &a.grid));
   }
   OSY_HIP_CHECK(hipStreamSynchronize(b->stream));
 
@@ -230,7 +230,7 @@ int rebuild(osy_batch* b) {
   hipGraph_t graph = nullptr;
   OSY_HIP_CHECK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
   for (const osy_batch::Group& g : b->groups)
-    enqueue_batch_step(g.storage, g.pressure, rows, g.args, b->stream);
+    enqueue_batch_step(g.storage, g.pressure, rows, b->occupancy, g.args, b->stream);
   hipError_t le = hipGetLastError();
   hipError_t ce = hipStreamEndCapture(b->stream, &graph);
   if (le != hipSuccess || ce != hipSuccess) {
@@ -268,8 +268,9 @@ int enqueue_steps(osy_batch* b, int32_t steps, hipStream_t s) {
   return OSY_OK;
 }
 
-bool tuning_ok(int rows, int k) {
-  return batch_rows_supported(rows) && k >= 1 && 8 * rows - 4 * k >= 1 && 128 - 4 * k >= 2;
+bool tuning_ok(int rows, int k, int occupancy) {
+  return batch_rows_supported(rows) && batch_occupancy_supported(rows, occupancy) && k >= 1 &&
+         8 * rows - 4 * k >= 1 && 128 - 4 * k >= 2;
 }
 
 }  // namespace
@@ -371,13 +372,16 @@ int osy_batch_synchronize(osy_batch* b) {
 
 int osy_batch_set_tuning(osy_batch* b, const osy_batch_tuning* t) {
   if (!b || !t) return set_error(OSY_E_INVALID, "null argument");
-  if (!tuning_ok(t->rows, t->sweeps_per_tile))
+  const int occ = t->occupancy == 0 ? 1 : t->occupancy;
+  if (!tuning_ok(t->rows, t->sweeps_per_tile, occ))
     return set_error(OSY_E_INVALID,
-                     "batch tuning needs rows 4 or 8 and 1 <= sweeps_per_tile < 2 * rows");
+                     "batch tuning needs rows 4 or 8, 1 <= sweeps_per_tile < 2 * rows, and "
+                     "occupancy 1 (or 2 with rows 8)");
   int st = sync_batch(b);
   if (st != OSY_OK) return st;
   b->rows = t->rows;
   b->k = t->sweeps_per_tile;
+  b->occupancy = occ;
   b->release();
   return OSY_OK;
 }
@@ -386,6 +390,7 @@ int osy_batch_get_tuning(const osy_batch* b, osy_batch_tuning* t) {
   if (!b || !t) return set_error(OSY_E_INVALID, "null argument");
   t->rows = b->rows;
   t->sweeps_per_tile = b->k;
+  t->occupancy = b->occupancy;
   return OSY_OK;
 }
 
@@ -410,18 +415,21 @@ int osy_batch_autotune(osy_batch* b, osy_batch_tuning* chosen, double* ms) {
     off += fl->block_bytes;
     parity.emplace_back(fl->uv, fl->sm);
   }
-  const int rows0 = b->rows, k0 = b->k;
-  int best_rows = rows0, best_k = k0;
+  const int rows0 = b->rows, k0 = b->k, occ0 = b->occupancy;
+  int best_rows = rows0, best_k = k0, best_occ = occ0;
   double best = 1e30;
   hipEvent_t a = nullptr, z = nullptr;
   if (e == hipSuccess) e = hipEventCreate(&a);
   if (e == hipSuccess) e = hipEventCreate(&z);
   st = e == hipSuccess ? OSY_OK : set_error(OSY_E_HIP, hipGetErrorString(e));
-  for (int rows : {4, 8}) {
+  const int variants[3][2] = {{4, 1}, {8, 1}, {8, 2}};
+  for (const auto& var : variants) {
+    const int rows = var[0], occ = var[1];
     for (int k = 1; k <= 12 && st == OSY_OK; k++) {
-      if (!tuning_ok(rows, k)) continue;
+      if (!tuning_ok(rows, k, occ)) continue;
       b->rows = rows;
       b->k = k;
+      b->occupancy = occ;
       b->release();
       st = enqueue_steps(b, 1, b->stream);  // warm-up (and rebuild)
       if (st != OSY_OK) break;
@@ -438,6 +446,7 @@ int osy_batch_autotune(osy_batch* b, osy_batch_tuning* chosen, double* ms) {
         best = t / reps;
         best_rows = rows;
         best_k = k;
+        best_occ = occ;
       }
     }
   }
@@ -461,11 +470,13 @@ int osy_batch_autotune(osy_batch* b, osy_batch_tuning* chosen, double* ms) {
   if (z) (void)hipEventDestroy(z);
   b->rows = st == OSY_OK ? best_rows : rows0;
   b->k = st == OSY_OK ? best_k : k0;
+  b->occupancy = st == OSY_OK ? best_occ : occ0;
   b->release();
   if (st != OSY_OK) return st;
   if (chosen) {
     chosen->rows = b->rows;
     chosen->sweeps_per_tile = b->k;
+    chosen->occupancy = b->occupancy;
   }
   if (ms) *ms = best;
   return OSY_OK;

@@ -61,6 +61,11 @@ template <> struct Store<__hip_bfloat16> {
 };
 
 template <typename T> __device__ inline float ldf(const T* p, long k) { return Store<T>::ld(p[k]); }
+// A value as the storage type holds it: the identity for float, round to
+// nearest even for the 16-bit types.  Every phase rounds the faces it
+// updates (as a launch per phase storing into 16-bit arrays would), so
+// 16-bit results do not depend on how sweeps are grouped into launches.
+template <typename T> __device__ inline float rnd(float x) { return Store<T>::ld(Store<T>::st(x)); }
 template <typename T> __device__ inline void stf(T* p, long k, float x) { p[k] = Store<T>::st(x); }
 
 // Grid geometry and the flag array, passed by value to every kernel.

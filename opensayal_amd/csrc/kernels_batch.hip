@@ -27,6 +27,14 @@ union BatchShared {
   AdvShared<kBatchAdvTY> adv;
 };
 
+// A value every lane holds the same copy of, as a scalar (so the claim, the
+// loop exit and the epoch arithmetic are wavefront-uniform).
+__device__ inline unsigned long long uniform_u64(unsigned long long x) {
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)x);
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(x >> 32));
+  return ((unsigned long long)hi << 32) | lo;
+}
+
 // Thread 0: wait until *counter >= target, polling with relaxed loads (an
 // acquire per poll would invalidate the L2 each time), then one agent-scope
 // acquire.  Returns false on timeout, or at once when an earlier wait of the
@@ -49,8 +57,8 @@ __device__ inline bool wait_count(const unsigned long long* counter, unsigned lo
 }
 
 template <typename T, bool PRESSURE, int R>
-__device__ inline void run_item(const BatchArgs& A, const BatchStage& st, const BatchSim& sim,
-                                int tile, unsigned epoch_bit, BatchShared& sh) {
+__device__ __attribute__((always_inline)) inline void run_item(const BatchStage& st, const BatchSim& sim, int tile,
+                                unsigned epoch_bit, int tw, int th, int k, BatchShared& sh) {
   const StepParams& sp = sim.sp;
   const int uv = sim.uv0 ^ (int)(epoch_bit & (unsigned)sim.flip_uv);
   const int sm = sim.sm0 ^ (int)(epoch_bit & (unsigned)sim.flip_sm);
@@ -58,9 +66,9 @@ __device__ inline void run_item(const BatchArgs& A, const BatchStage& st, const 
   const int kind = st.kind;
   if (kind == BK_PROJ) {
     RegArgs a;
-    a.tw = A.tw;
-    a.th = A.th;
-    a.k = A.k;
+    a.tw = tw;
+    a.th = th;
+    a.k = k;
     a.ntx = st.ntx;
     a.sweeps = st.sweeps;
     a.forces = st.forces;
@@ -105,8 +113,10 @@ __device__ inline void run_item(const BatchArgs& A, const BatchStage& st, const 
 // completion count of the current one.  (Two thread-0 regions adjacent across
 // the loop's back-edge let the compiler merge them and move a barrier into a
 // loop that thread 0 leaves, which deadlocks the other threads.)
-template <typename T, bool PRESSURE, int R>
-__global__ void __launch_bounds__(kBatchThreads) k_batch_step(BatchArgs A) {
+// OCC2: registers capped for two workgroups per CU (at the price of some
+// spilling for R = 8); the batch autotuner decides whether that pays.
+template <typename T, bool PRESSURE, int R, bool OCC2>
+__global__ void __launch_bounds__(kBatchThreads, OCC2 ? 4 : 1) k_batch_step(BatchArgs A) {
   __shared__ BatchShared sh;
   __shared__ unsigned long long s_claim;
   __shared__ int s_ok;
@@ -114,10 +124,10 @@ __global__ void __launch_bounds__(kBatchThreads) k_batch_step(BatchArgs A) {
   if (threadIdx.x == 0)
     s_claim = __hip_atomic_fetch_add(A.head, 1ULL, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   __syncthreads();
-  unsigned long long raw = s_claim;
+  unsigned long long raw = uniform_u64(s_claim);
   for (;;) {
     const unsigned long long epoch = raw / per;
-    const int idx = __builtin_amdgcn_readfirstlane((int)(raw - epoch * per));
+    const int idx = (int)(raw - epoch * per);
     if (idx >= A.nitems) break;
     const uint2 it = A.items[idx];
     const int sid = __builtin_amdgcn_readfirstlane((int)it.x);
@@ -137,7 +147,8 @@ __global__ void __launch_bounds__(kBatchThreads) k_batch_step(BatchArgs A) {
     __syncthreads();
     if (__builtin_amdgcn_readfirstlane(s_ok)) {
       const int simi = __builtin_amdgcn_readfirstlane(st.sim);
-      run_item<T, PRESSURE, R>(A, st, A.sims[simi], tile, (unsigned)(epoch & 1ULL), sh);
+      run_item<T, PRESSURE, R>(st, A.sims[simi], tile, (unsigned)(epoch & 1ULL), A.tw, A.th, A.k,
+                               sh);
     }
     // every wavefront's stores have reached the L2 before the barrier; thread
     // 0's release then writes the L2 back once for the whole workgroup
@@ -148,44 +159,48 @@ __global__ void __launch_bounds__(kBatchThreads) k_batch_step(BatchArgs A) {
       s_claim = __hip_atomic_fetch_add(A.head, 1ULL, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     __syncthreads();
-    raw = s_claim;
+    raw = uniform_u64(s_claim);
   }
 }
 
 template <typename T, bool PRESSURE>
-const void* batch_kernel(int rows) {
-  return rows == 4 ? (const void*)k_batch_step<T, PRESSURE, 4>
-                   : (const void*)k_batch_step<T, PRESSURE, 8>;
+const void* batch_kernel(int rows, int occupancy) {
+  if (rows == 4) return (const void*)k_batch_step<T, PRESSURE, 4, false>;
+  return occupancy == 2 ? (const void*)k_batch_step<T, PRESSURE, 8, true>
+                        : (const void*)k_batch_step<T, PRESSURE, 8, false>;
 }
 
-const void* batch_kernel_for(int storage, bool pressure, int rows) {
+const void* batch_kernel_for(int storage, bool pressure, int rows, int occupancy) {
   switch (storage) {
     case OSY_STORAGE_F16:
-      return pressure ? batch_kernel<__half, true>(rows) : batch_kernel<__half, false>(rows);
+      return pressure ? batch_kernel<__half, true>(rows, occupancy)
+                      : batch_kernel<__half, false>(rows, occupancy);
     case OSY_STORAGE_BF16:
-      return pressure ? batch_kernel<__hip_bfloat16, true>(rows)
-                      : batch_kernel<__hip_bfloat16, false>(rows);
+      return pressure ? batch_kernel<__hip_bfloat16, true>(rows, occupancy)
+                      : batch_kernel<__hip_bfloat16, false>(rows, occupancy);
     default:
-      return pressure ? batch_kernel<float, true>(rows) : batch_kernel<float, false>(rows);
+      return pressure ? batch_kernel<float, true>(rows, occupancy)
+                      : batch_kernel<float, false>(rows, occupancy);
   }
 }
 
 }  // namespace
 
-hipError_t batch_grid_size(int storage, bool pressure, int rows, int* grid) {
+hipError_t batch_grid_size(int storage, bool pressure, int rows, int occupancy, int* grid) {
   int dev = 0, cus = 0, per_cu = 0;
   hipError_t e = hipGetDevice(&dev);
   if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   if (e == hipSuccess)
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_kernel_for(storage, pressure, rows),
-                                                     kBatchThreads, 0);
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &per_cu, batch_kernel_for(storage, pressure, rows, occupancy), kBatchThreads, 0);
   if (e != hipSuccess) return e;
   *grid = (per_cu < 1 ? 1 : per_cu) * (cus < 1 ? 1 : cus);
   return hipSuccess;
 }
 
-void enqueue_batch_step(int storage, bool pressure, int rows, const BatchArgs& a, hipStream_t s) {
-  const void* fn = batch_kernel_for(storage, pressure, rows);
+void enqueue_batch_step(int storage, bool pressure, int rows, int occupancy, const BatchArgs& a,
+                        hipStream_t s) {
+  const void* fn = batch_kernel_for(storage, pressure, rows, occupancy);
   void* args[] = {(void*)&a};
   (void)hipLaunchKernel(fn, dim3(a.grid), dim3(kBatchThreads), args, 0, s);
 }

@@ -138,9 +138,13 @@ __global__ void __launch_bounds__(kTileThreads)
     const int l0 = r * rs + q, l1 = l0 + pwp;
     load_pair(u_in, gk, in0, in1, a0, a1);
     load_pair(v_in, gk, in0, in1, b0, b1);
-    if (a.forces) {
+    if (a.forces) {  // the forces' results as stored (rounded to T)
       if (in0) force_uv(sp, gi, gj, a0, b0);
       if (in1) force_uv(sp, gi + 1, gj, a1, b1);
+      a0 = rnd<T>(a0);
+      a1 = rnd<T>(a1);
+      b0 = rnd<T>(b0);
+      b1 = rnd<T>(b1);
     }
     U[l0] = a0;
     U[l1] = a1;
@@ -208,10 +212,10 @@ __global__ void __launch_bounds__(kTileThreads)
         if constexpr (PRESSURE) {
           if (g & G_UPD) S[own] = S[own] + (((d * sp.density) * (float)sp.cs) / sp.d_t);
         }
-        U[own] = gated_add(uu, d, g, 0);
-        U[right] = gated_sub(ur, d, g, 1);
-        V[own] = gated_add(vv, d, g, 2);
-        V[top] = gated_sub(vt, d, g, 3);
+        U[own] = rnd<T>(gated_add(uu, d, g, 0));
+        U[right] = rnd<T>(gated_sub(ur, d, g, 1));
+        V[own] = rnd<T>(gated_add(vv, d, g, 2));
+        V[top] = rnd<T>(gated_sub(vt, d, g, 3));
       }
       own += bstep ? (b ? rstep * rs - pwp : rstep * rs + pwp) : rstep * rs;
       b ^= bstep;

@@ -166,8 +166,8 @@ struct RegTile {
 
 // apply_projection_at for the active cell of row rho (parity B) of every
 // lane.  mr masks rows outside the box, m0/m1 lanes outside it.
-template <int R, int B, bool PRESSURE>
-__device__ inline void reg_row(RegTile<R>& t, const int rho, uint32_t mr, uint32_t m0,
+template <typename T, int R, int B, bool PRESSURE>
+__device__ __attribute__((always_inline)) inline void reg_row(RegTile<R>& t, const int rho, uint32_t mr, uint32_t m0,
                                uint32_t m1, float vt0, float vt1, float& dtop, float o,
                                float density, float csf, float d_t) {
   const int up = rho + 1 < R ? rho + 1 : rho;  // row above (when inside the strip)
@@ -181,11 +181,11 @@ __device__ inline void reg_row(RegTile<R>& t, const int rho, uint32_t mr, uint32
     if constexpr (PRESSURE) {
       if (g & G_UPD) t.s0[rho] = t.s0[rho] + pressure_term(d, density, csf, d_t);
     }
-    t.u0[rho] = gated_add(uu, d, g, 0);
-    t.u1[rho] = gated_sub(ur, d, g, 1);
-    t.v0[rho] = gated_add(vv, d, g, 2);
+    t.u0[rho] = rnd<T>(gated_add(uu, d, g, 0));
+    t.u1[rho] = rnd<T>(gated_sub(ur, d, g, 1));
+    t.v0[rho] = rnd<T>(gated_add(vv, d, g, 2));
     if (rho + 1 < R)
-      t.v0[up] = gated_sub(vt, d, g, 3);
+      t.v0[up] = rnd<T>(gated_sub(vt, d, g, 3));
     else
       dtop = gate_pos(d, g, 3);
   } else {
@@ -198,26 +198,26 @@ __device__ inline void reg_row(RegTile<R>& t, const int rho, uint32_t mr, uint32
     if constexpr (PRESSURE) {
       if (g & G_UPD) t.s1[rho] = t.s1[rho] + pressure_term(d, density, csf, d_t);
     }
-    t.u1[rho] = gated_add(uu, d, g, 0);
-    t.u0[rho] = t.u0[rho] - lane_from_left(gate_pos(d, g, 1));
-    t.v1[rho] = gated_add(vv, d, g, 2);
+    t.u1[rho] = rnd<T>(gated_add(uu, d, g, 0));
+    t.u0[rho] = rnd<T>(t.u0[rho] - lane_from_left(gate_pos(d, g, 1)));
+    t.v1[rho] = rnd<T>(gated_add(vv, d, g, 2));
     if (rho + 1 < R)
-      t.v1[up] = gated_sub(vt, d, g, 3);
+      t.v1[up] = rnd<T>(gated_sub(vt, d, g, 3));
     else
       dtop = gate_pos(d, g, 3);
   }
 }
 
 // Rows RHO..R-1 of one phase, unrolled at compile time.
-template <int R, int PB, bool PRESSURE, int RHO>
-__device__ inline void reg_rows_from(RegTile<R>& t, int rlo, int rhi_rel, uint32_t m0, uint32_t m1,
+template <typename T, int R, int PB, bool PRESSURE, int RHO>
+__device__ __attribute__((always_inline)) inline void reg_rows_from(RegTile<R>& t, int rlo, int rhi_rel, uint32_t m0, uint32_t m1,
                                      float vt0, float vt1, float& dtop, float o, float density,
                                      float csf, float d_t) {
   if constexpr (RHO < R) {
     const uint32_t mr = (RHO >= rlo && RHO < rhi_rel) ? 0xffu : 0u;  // wavefront-uniform
-    reg_row<R, ((PB + RHO) & 1), PRESSURE>(t, RHO, mr, m0, m1, vt0, vt1, dtop, o, density, csf,
+    reg_row<T, R, ((PB + RHO) & 1), PRESSURE>(t, RHO, mr, m0, m1, vt0, vt1, dtop, o, density, csf,
                                            d_t);
-    reg_rows_from<R, PB, PRESSURE, RHO + 1>(t, rlo, rhi_rel, m0, m1, vt0, vt1, dtop, o, density,
+    reg_rows_from<T, R, PB, PRESSURE, RHO + 1>(t, rlo, rhi_rel, m0, m1, vt0, vt1, dtop, o, density,
                                             csf, d_t);
   }
 }
@@ -226,8 +226,8 @@ __device__ inline void reg_rows_from(RegTile<R>& t, int rlo, int rhi_rel, uint32
 // wavefront (compile time: phases come in pairs with PB and 1 - PB).  Rows
 // outside the box of cells that still reach the tile are masked rather
 // than skipped, so the phase is straight-line code over registers.
-template <int R, int PB, bool PRESSURE>
-__device__ inline void reg_phase(RegTile<R>& t, float2 (*vrow)[64], float (*dsend)[64],
+template <typename T, int R, int PB, bool PRESSURE>
+__device__ __attribute__((always_inline)) inline void reg_phase(RegTile<R>& t, float2 (*vrow)[64], float (*dsend)[64],
                                  int lane, int w, int rbase, int s, int nphase, int K, int tw,
                                  int th, float o, float density, float csf, float d_t) {
   const int e = nphase - 1 - s;
@@ -249,7 +249,7 @@ __device__ inline void reg_phase(RegTile<R>& t, float2 (*vrow)[64], float (*dsen
   }
   float dtop = 0.0f;
   if (rlo < R && rhi_rel > 0)  // wavefront-uniform: some row is in the box
-    reg_rows_from<R, PB, PRESSURE, 0>(t, rlo, rhi_rel, m0, m1, vt0, vt1, dtop, o, density, csf,
+    reg_rows_from<T, R, PB, PRESSURE, 0>(t, rlo, rhi_rel, m0, m1, vt0, vt1, dtop, o, density, csf,
                                       d_t);
   if (w < kRegWaves - 1) dsend[w][lane] = dtop;
   __syncthreads();
@@ -258,9 +258,9 @@ __device__ inline void reg_phase(RegTile<R>& t, float2 (*vrow)[64], float (*dsen
     // this wavefront's row-0 v of parity 1 - PB
     const float dd = dsend[w - 1][lane];
     if constexpr (PB == 0)
-      t.v1[0] = t.v1[0] - dd;
+      t.v1[0] = rnd<T>(t.v1[0] - dd);
     else
-      t.v0[0] = t.v0[0] - dd;
+      t.v0[0] = rnd<T>(t.v0[0] - dd);
   }
 }
 
@@ -274,7 +274,7 @@ struct RegShared {
 // over the grid's ntx tiles per row), run by a workgroup of 64 * kRegWaves
 // threads.
 template <typename T, bool PRESSURE, int R>
-__device__ inline void project_regs_tile(const StepParams& sp, const uint8_t* __restrict__ flags,
+__device__ __attribute__((always_inline)) inline void project_regs_tile(const StepParams& sp, const uint8_t* __restrict__ flags,
                                          const T* __restrict__ u_in, const T* __restrict__ v_in,
                                          T* __restrict__ u_out, T* __restrict__ v_out,
                                          float* __restrict__ p, const RegArgs& a, int tile,
@@ -316,9 +316,17 @@ __device__ inline void project_regs_tile(const StepParams& sp, const uint8_t* __
     const long gk = (long)gj * P + gi;
     load_pair(u_in, gk, row_in && col0, row_in && col1, t.u0[rho], t.u1[rho]);
     load_pair(v_in, gk, row_in && col0, row_in && col1, t.v0[rho], t.v1[rho]);
-    if (a.forces) {
-      if (row_in && col0) force_uv(sp, gi, gj, t.u0[rho], t.v0[rho]);
-      if (row_in && col1) force_uv(sp, gi + 1, gj, t.u1[rho], t.v1[rho]);
+    if (a.forces) {  // the forces' results as stored (rounded to T)
+      if (row_in && col0) {
+        force_uv(sp, gi, gj, t.u0[rho], t.v0[rho]);
+        t.u0[rho] = rnd<T>(t.u0[rho]);
+        t.v0[rho] = rnd<T>(t.v0[rho]);
+      }
+      if (row_in && col1) {
+        force_uv(sp, gi + 1, gj, t.u1[rho], t.v1[rho]);
+        t.u1[rho] = rnd<T>(t.u1[rho]);
+        t.v1[rho] = rnd<T>(t.v1[rho]);
+      }
     }
     if constexpr (PRESSURE) {
       t.s0[rho] = (row_in && col0 && !a.forces) ? p[gk] : 0.0f;
@@ -339,16 +347,16 @@ __device__ inline void project_regs_tile(const StepParams& sp, const uint8_t* __
   // is (gy0 + rbase) & 1 and flips every phase
   if (((gy0 + rbase) & 1) == 0) {
     for (int s = 0; s < nphase; s += 2) {
-      reg_phase<R, 0, PRESSURE>(t, vrow, dsend, lane, w, rbase, s, nphase, K, a.tw, a.th, o,
+      reg_phase<T, R, 0, PRESSURE>(t, vrow, dsend, lane, w, rbase, s, nphase, K, a.tw, a.th, o,
                                 sp.density, csf, sp.d_t);
-      reg_phase<R, 1, PRESSURE>(t, vrow, dsend, lane, w, rbase, s + 1, nphase, K, a.tw, a.th, o,
+      reg_phase<T, R, 1, PRESSURE>(t, vrow, dsend, lane, w, rbase, s + 1, nphase, K, a.tw, a.th, o,
                                 sp.density, csf, sp.d_t);
     }
   } else {
     for (int s = 0; s < nphase; s += 2) {
-      reg_phase<R, 1, PRESSURE>(t, vrow, dsend, lane, w, rbase, s, nphase, K, a.tw, a.th, o,
+      reg_phase<T, R, 1, PRESSURE>(t, vrow, dsend, lane, w, rbase, s, nphase, K, a.tw, a.th, o,
                                 sp.density, csf, sp.d_t);
-      reg_phase<R, 0, PRESSURE>(t, vrow, dsend, lane, w, rbase, s + 1, nphase, K, a.tw, a.th, o,
+      reg_phase<T, R, 0, PRESSURE>(t, vrow, dsend, lane, w, rbase, s + 1, nphase, K, a.tw, a.th, o,
                                 sp.density, csf, sp.d_t);
     }
   }

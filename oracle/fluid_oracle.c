@@ -36,6 +36,47 @@ static inline int orc_valid(const orc_grid* g, int i, int j) {
   return i >= 0 && j >= 0 && i < g->W && j < g->H && !g->solid[orc_idx(g, i, j)];
 }
 
+/* ---- 16-bit storage (sim.storage f16 / bf16) ----------------------------
+ * The state is held in the storage type: every phase's outputs are rounded
+ * to it (round to nearest, ties to even), as if each of the reference's
+ * kernels stored into 16-bit arrays; arithmetic stays single precision.
+ * Only the single-precision oracle rounds; the double one is the FP64
+ * reference run. */
+static float orc_round16(float x, int storage) {
+  uint32_t b;
+  memcpy(&b, &x, 4);
+  const uint32_t a = b & 0x7fffffffu;
+  if (a > 0x7f800000u) return x; /* NaN */
+  if (storage == OSY_STORAGE_BF16) {
+    b = (b + 0x7fffu + ((b >> 16) & 1u)) & 0xffff0000u;
+    memcpy(&x, &b, 4);
+    return x;
+  }
+  /* binary16 */
+  if (a >= 0x477ff000u) return copysignf(INFINITY, x); /* |x| >= 65520 */
+  if (a < 0x38800000u) {                               /* |x| < 2^-14: spacing 2^-24 */
+    const float r = x >= 0.0f ? (x + 0.5f) - 0.5f : (x - 0.5f) + 0.5f;
+    return copysignf(r, x);
+  }
+  b = (b + 0x0fffu + ((b >> 13) & 1u)) & 0xffffe000u;
+  memcpy(&x, &b, 4);
+  return x;
+}
+
+static void orc_round_f32(float* a, long n, int storage) {
+  if (storage != OSY_STORAGE_F16 && storage != OSY_STORAGE_BF16) return;
+  for (long k = 0; k < n; k++) a[k] = orc_round16(a[k], storage);
+}
+
+static void orc_round_f64(double* a, long n, int storage) {
+  (void)a;
+  (void)n;
+  (void)storage;
+}
+
+/* Rounding of an array in place, for the tests of the rounding itself. */
+void orc_round16_array(float* a, long n, int storage) { orc_round_f32(a, n, storage); }
+
 /* ---- single precision: the reference's arithmetic ---------------------- */
 #define REAL float
 #define SFX _f32

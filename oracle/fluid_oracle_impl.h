@@ -250,9 +250,13 @@ void FN(orc_projection_phase)(const osy_config* c, const orc_grid* g, REAL* u,
 /* apply_projection (src/fluid.cu:282-295): `sweeps` times even then odd. */
 void FN(orc_projection)(const osy_config* c, const orc_grid* g, REAL* u,
                         REAL* v, REAL* p, REAL d_t, int sweeps, int nthreads) {
+  const long n = (long)g->W * g->H;
   for (int s = 0; s < sweeps; s++) {
-    FN(orc_projection_phase)(c, g, u, v, p, d_t, 0, nthreads);
-    FN(orc_projection_phase)(c, g, u, v, p, d_t, 1, nthreads);
+    for (int colour = 0; colour < 2; colour++) {
+      FN(orc_projection_phase)(c, g, u, v, p, d_t, colour, nthreads);
+      FN(orc_round)(u, n, c->storage);
+      FN(orc_round)(v, n, c->storage);
+    }
   }
 }
 
@@ -377,7 +381,14 @@ void FN(orc_update)(const osy_config* c, const orc_grid* g, REAL* u, REAL* v,
                     REAL* smoke, REAL* p, REAL* scratch, REAL* prange,
                     const osy_source* src, REAL d_t, int nthreads) {
   const long n = (long)g->W * g->H;
+  /* the state as the storage type holds it, then the forces' stores */
+  FN(orc_round)(u, n, c->storage);
+  FN(orc_round)(v, n, c->storage);
+  FN(orc_round)(smoke, n, c->storage);
   FN(orc_forces)(c, g, u, v, smoke, p, src, d_t, nthreads);
+  FN(orc_round)(u, n, c->storage);
+  FN(orc_round)(v, n, c->storage);
+  FN(orc_round)(smoke, n, c->storage);
   FN(orc_projection)(c, g, u, v, p, d_t, c->projection_n, nthreads);
   if (c->enable_pressure && prange) {
     REAL lo = (REAL)INFINITY, hi = (REAL)-INFINITY;
@@ -392,13 +403,19 @@ void FN(orc_update)(const osy_config* c, const orc_grid* g, REAL* u, REAL* v,
   REAL* u2 = scratch;
   REAL* v2 = scratch + n;
   FN(orc_advect_velocity)(g, u, v, u2, v2, d_t, nthreads);
+  FN(orc_round)(u2, n, c->storage);
+  FN(orc_round)(v2, n, c->storage);
   memcpy(u, u2, sizeof(REAL) * n);
   memcpy(v, v2, sizeof(REAL) * n);
   if (c->enable_smoke && c->wind_smoke != 0) {
     REAL* s2 = scratch + 2 * n;
     FN(orc_advect_smoke)(g, u, v, smoke, s2, d_t, nthreads);
+    FN(orc_round)(s2, n, c->storage);
     memcpy(smoke, s2, sizeof(REAL) * n);
-    if (c->enable_decay) FN(orc_decay)(g, smoke, (REAL)c->decay_rate, d_t, nthreads);
+    if (c->enable_decay) {
+      FN(orc_decay)(g, smoke, (REAL)c->decay_rate, d_t, nthreads);
+      FN(orc_round)(smoke, n, c->storage);
+    }
   }
 }
 

@@ -79,8 +79,20 @@ def _declare(L):
         f.argtypes = [_P, _P, _P, _P, _P, _P]
 
 
+    L.orc_round16_array.restype = None
+    L.orc_round16_array.argtypes = [_P, ctypes.c_long, ctypes.c_int]
+
+
 def _ptr(a):
     return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def round16(a: np.ndarray, storage: int) -> np.ndarray:
+    """float32 values rounded to binary16 (storage 1) or bfloat16 (2), ties
+    to even, widened back; the oracle's rounding for 16-bit storage."""
+    out = np.ascontiguousarray(a, dtype=np.float32).copy()
+    lib().orc_round16_array(_ptr(out), out.size, storage)
+    return out
 
 
 def build_masks(cfg):

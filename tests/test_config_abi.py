@@ -29,14 +29,14 @@ def test_library_exports_every_declared_function():
     for name in names:
         assert hasattr(lib, name), name
         assert name in _abi.PROTOTYPES, name
-    assert lib.osy_abi_version() == 2
+    assert lib.osy_abi_version() == 3
 
 
 def test_struct_sizes_match_header_layout():
     # 34 four-byte fields, no padding
     assert ctypes.sizeof(_abi.OsyConfig) == 34 * 4
     assert ctypes.sizeof(_abi.OsySource) == 5 * 4
-    assert ctypes.sizeof(_abi.OsyBatchTuning) == 2 * 4
+    assert ctypes.sizeof(_abi.OsyBatchTuning) == 3 * 4
     # count, groups, then four 8-byte fields
     assert ctypes.sizeof(_abi.OsyBatchInfo) == 8 + 4 * 8
 

@@ -98,22 +98,23 @@ def test_batch_member_matches_oracle():
         assert np.array_equal(fl.get_field(name).view(np.uint32), getattr(o, attr).view(np.uint32))
 
 
-SHAPES = [(4, 1), (4, 3), (4, 7), (8, 2), (8, 5), (8, 9), (8, 12)]
+SHAPES = [(4, 1, 1), (4, 3, 1), (4, 7, 1), (8, 2, 1), (8, 5, 1), (8, 9, 1), (8, 12, 1),
+          (8, 3, 2), (8, 6, 2)]
 
 
-@pytest.mark.parametrize("rows,k", SHAPES)
-def test_batch_shapes_equal_alone(rows, k):
-    cfgs = mixed_configs()[:7]
+@pytest.mark.parametrize("rows,k,occ", SHAPES)
+def test_batch_shapes_equal_alone(rows, k, occ):
+    cfgs = mixed_configs()[:7] + mixed_configs()[8:11]
     batched, alone = pairs(cfgs, 300)
     with Batch() as b:
-        b.set_tuning(rows=rows, sweeps_per_tile=k)
+        b.set_tuning(rows=rows, sweeps_per_tile=k, occupancy=occ)
         for fl in batched:
             b.add(fl)
         b.step(3)
         for fl in alone:
             fl.step(3)
         for i, (x, y) in enumerate(zip(batched, alone)):
-            assert_same(state(x), state(y), f"rows={rows} k={k} sim {i}")
+            assert_same(state(x), state(y), f"rows={rows} k={k} occupancy={occ} sim {i}")
 
 
 def test_membership_changes_keep_results():
@@ -163,7 +164,7 @@ def test_autotune_restores_state():
             b.add(fl)
         before = [state(fl) for fl in batched]
         tuning, ms = b.autotune()
-        assert ms > 0 and tuning["rows"] in (4, 8)
+        assert ms > 0 and tuning["rows"] in (4, 8) and tuning["occupancy"] in (1, 2)
         for fl, s in zip(batched, before):
             assert_same(state(fl), s, "after autotune")
         b.step(2)
@@ -190,6 +191,8 @@ def test_budget_and_state_errors():
             f1.step(1)  # stepped by the batch
         with pytest.raises(OpenSayalError):
             b.set_tuning(rows=16, sweeps_per_tile=2)
+        with pytest.raises(OpenSayalError):
+            b.set_tuning(rows=4, sweeps_per_tile=2, occupancy=2)
         b.remove(f2)
         b.add(f3)
         b.step(1)

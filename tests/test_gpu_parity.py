@@ -297,3 +297,29 @@ def test_library_stream_and_events():
     for n, _ in FIELDS:
         assert np.array_equal(a.get_field(n), b.get_field(n))
     s.close()
+
+
+@pytest.mark.parametrize("storage", ["f16", "bf16"])
+def test_half_storage_bit_exact_with_oracle(storage):
+    """16-bit storage: every phase's outputs rounded to the storage type,
+    FP32 arithmetic; the oracle's storage mode gives the same bits."""
+    cfg = wl.plain_small(storage=storage, pressure=True)
+    fl, o = pair(cfg, seed=16)
+    fl.step(5)
+    o.update(steps=5)
+    assert_bitwise(fl, o, f"{storage} 5 steps")
+
+
+@pytest.mark.parametrize("t", [0, 1, 2, 7, 8])
+@pytest.mark.parametrize("storage", [1, 2])
+def test_half_storage_shapes_bit_exact(t, storage):
+    """Rounding per phase makes 16-bit results independent of the launch
+    shape (sweeps per launch, tile), so every shape matches the oracle."""
+    cfg = make_config(77, 53, projection_n=13, wind_speed=2.0, g=-5.0, obstacle_center_x=30,
+                      obstacle_center_y=20, obstacle_radius=8.5, storage=storage,
+                      enable_decay=1, decay_rate=0.2)
+    fl, o = pair(cfg, seed=17)
+    fl.set_tuning(**TUNINGS[t])
+    fl.step(3)
+    o.update(steps=3)
+    assert_bitwise(fl, o, f"storage {storage} {TUNINGS[t]}")
