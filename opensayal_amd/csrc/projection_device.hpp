@@ -165,15 +165,18 @@ struct RegTile {
 };
 
 // apply_projection_at for the active cell of row rho (parity B) of every
-// lane.  mr masks rows outside the box, m0/m1 lanes outside it.
+// lane.  Cells near the region edge compute with stale neighbours; their
+// wrong values move one cell per phase and never reach the tile within a
+// launch, so no lane or row is masked.
 template <typename T, int R, int B, bool PRESSURE>
-__device__ __attribute__((always_inline)) inline void reg_row(RegTile<R>& t, const int rho, uint32_t mr, uint32_t m0,
-                               uint32_t m1, float vt0, float vt1, float& dtop, float o,
-                               float density, float csf, float d_t) {
+__device__ __attribute__((always_inline)) inline void reg_row(RegTile<R>& t, const int rho,
+                                                              float vt0, float vt1, float& dtop,
+                                                              float o, float density, float csf,
+                                                              float d_t) {
   const int up = rho + 1 < R ? rho + 1 : rho;  // row above (when inside the strip)
   if constexpr (B == 0) {
     // active cell 2l: faces u0, u1, v0[rho], v0[rho+1] (all in this lane)
-    const uint32_t g = t.gw[rho] & m0 & mr;
+    const uint32_t g = t.gw[rho];
     const float uu = t.u0[rho], ur = t.u1[rho], vv = t.v0[rho];
     const float vt = (rho + 1 < R) ? t.v0[up] : vt0;
     const float div = ((ur - uu) + vt) - vv;
@@ -190,7 +193,7 @@ __device__ __attribute__((always_inline)) inline void reg_row(RegTile<R>& t, con
       dtop = gate_pos(d, g, 3);
   } else {
     // active cell 2l+1: faces u1, u0 of lane l+1, v1[rho], v1[rho+1]
-    const uint32_t g = (t.gw[rho] >> 8) & m1 & mr;
+    const uint32_t g = t.gw[rho] >> 8;
     const float uu = t.u1[rho], ur = lane_from_right(t.u0[rho]), vv = t.v1[rho];
     const float vt = (rho + 1 < R) ? t.v1[up] : vt1;
     const float div = ((ur - uu) + vt) - vv;
@@ -210,33 +213,31 @@ __device__ __attribute__((always_inline)) inline void reg_row(RegTile<R>& t, con
 
 // Rows RHO..R-1 of one phase, unrolled at compile time.
 template <typename T, int R, int PB, bool PRESSURE, int RHO>
-__device__ __attribute__((always_inline)) inline void reg_rows_from(RegTile<R>& t, int rlo, int rhi_rel, uint32_t m0, uint32_t m1,
-                                     float vt0, float vt1, float& dtop, float o, float density,
-                                     float csf, float d_t) {
+__device__ __attribute__((always_inline)) inline void reg_rows_from(RegTile<R>& t, float vt0,
+                                                                    float vt1, float& dtop,
+                                                                    float o, float density,
+                                                                    float csf, float d_t) {
   if constexpr (RHO < R) {
-    const uint32_t mr = (RHO >= rlo && RHO < rhi_rel) ? 0xffu : 0u;  // wavefront-uniform
-    reg_row<T, R, ((PB + RHO) & 1), PRESSURE>(t, RHO, mr, m0, m1, vt0, vt1, dtop, o, density, csf,
-                                           d_t);
-    reg_rows_from<T, R, PB, PRESSURE, RHO + 1>(t, rlo, rhi_rel, m0, m1, vt0, vt1, dtop, o, density,
-                                            csf, d_t);
+    reg_row<T, R, ((PB + RHO) & 1), PRESSURE>(t, RHO, vt0, vt1, dtop, o, density, csf, d_t);
+    reg_rows_from<T, R, PB, PRESSURE, RHO + 1>(t, vt0, vt1, dtop, o, density, csf, d_t);
   }
 }
 
 // One colour phase.  PB is the parity of the active cell in row 0 of this
-// wavefront (compile time: phases come in pairs with PB and 1 - PB).  Rows
-// outside the box of cells that still reach the tile are masked rather
-// than skipped, so the phase is straight-line code over registers.
+// wavefront (compile time: phases come in pairs with PB and 1 - PB).  A
+// wavefront none of whose rows can still reach the tile skips the phase;
+// the others run it as straight-line code over registers.
 template <typename T, int R, int PB, bool PRESSURE>
-__device__ __attribute__((always_inline)) inline void reg_phase(RegTile<R>& t, float2 (*vrow)[64], float (*dsend)[64],
-                                 int lane, int w, int rbase, int s, int nphase, int K, int tw,
-                                 int th, float o, float density, float csf, float d_t) {
+__device__ __attribute__((always_inline)) inline void reg_phase(RegTile<R>& t, float2 (*vrow)[64],
+                                                                float (*dsend)[64], int lane,
+                                                                int w, int rbase, int s,
+                                                                int nphase, int K, int tw,
+                                                                int th, float o, float density,
+                                                                float csf, float d_t) {
+  (void)tw;
   const int e = nphase - 1 - s;
   const int lo = 2 * K - 1 - e;
-  const int chi = 2 * K + tw + e;
   const int rhi = 2 * K + th + e;
-  const int lc = 2 * lane;
-  const uint32_t m0 = (lc >= lo && lc < chi) ? 0xffu : 0u;
-  const uint32_t m1 = (lc + 1 >= lo && lc + 1 < chi) ? 0xffu : 0u;
   const int rlo = lo - rbase, rhi_rel = rhi - rbase;  // box rows of this wavefront
   // publish row 0 for the wavefront below, read the row above
   if (w > 0) vrow[w][lane] = make_float2(t.v0[0], t.v1[0]);
@@ -248,9 +249,14 @@ __device__ __attribute__((always_inline)) inline void reg_phase(RegTile<R>& t, f
     vt1 = x.y;
   }
   float dtop = 0.0f;
-  if (rlo < R && rhi_rel > 0)  // wavefront-uniform: some row is in the box
-    reg_rows_from<T, R, PB, PRESSURE, 0>(t, rlo, rhi_rel, m0, m1, vt0, vt1, dtop, o, density, csf,
-                                      d_t);
+  if (rlo < R && rhi_rel > 0) {  // wavefront-uniform: some row is in the box
+    // the gate words do not change during the launch; hiding that from the
+    // compiler keeps it from hoisting the 8 face masks per row out of the
+    // phase loop into registers
+#pragma unroll
+    for (int rho = 0; rho < R; rho++) asm volatile("" : "+v"(t.gw[rho]));
+    reg_rows_from<T, R, PB, PRESSURE, 0>(t, vt0, vt1, dtop, o, density, csf, d_t);
+  }
   if (w < kRegWaves - 1) dsend[w][lane] = dtop;
   __syncthreads();
   if (w > 0) {

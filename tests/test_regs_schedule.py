@@ -68,7 +68,7 @@ def gate_mask(g, bit):
     return ((g >> bit) & 1).astype(bool)
 
 
-def emulate_launch(cfg, flags, u_in, v_in, p, tw, th, K, sweeps, R):
+def emulate_launch(cfg, flags, u_in, v_in, p, tw, th, K, sweeps, R, mask=True):
     H, W = u_in.shape
     assert tw == 128 - 4 * K and th + 4 * K == NWV * R
     ntx, nty = -(-W // tw), -(-H // th)
@@ -117,6 +117,8 @@ def emulate_launch(cfg, flags, u_in, v_in, p, tw, th, K, sweeps, R):
             lc = 2 * lanes
             m0 = np.where((lc >= lo) & (lc < chi), 0xFF, 0)
             m1 = np.where((lc + 1 >= lo) & (lc + 1 < chi), 0xFF, 0)
+            if not mask:  # every lane of a wavefront that has a row in the box updates
+                m0 = m1 = np.full(64, 0xFF)
             vrow = np.zeros((NWV, 2, 64), f32)
             for w in range(1, NWV):
                 vrow[w, 0], vrow[w, 1] = v0[w, 0], v1[w, 0]
@@ -128,9 +130,12 @@ def emulate_launch(cfg, flags, u_in, v_in, p, tw, th, K, sweeps, R):
                 pb = (c + gy0 + w * R) & 1
                 pbs.append(pb)
                 dtop = np.zeros(64, f32)
+                wave_in_box = lo - w * R < R and rhi - w * R > 0
                 for rho in range(R):
                     r = w * R + rho
-                    if r < lo or r >= rhi:
+                    if mask and (r < lo or r >= rhi):
+                        continue
+                    if not mask and not wave_in_box:
                         continue
                     b = (pb + rho) & 1
                     if b == 0:
@@ -196,31 +201,33 @@ def emulate_launch(cfg, flags, u_in, v_in, p, tw, th, K, sweeps, R):
     return u_out, v_out, p_out
 
 
-def emulate_projection(cfg, o, K, R):
+def emulate_projection(cfg, o, K, R, mask=True):
     flags = device_flags(o)
     tw, th = 128 - 4 * K, NWV * R - 4 * K
     u, v, p = o.u[::-1].copy(), o.v[::-1].copy(), o.p[::-1].copy()
     done = 0
     while done < cfg.projection_n:
         sw = min(K, cfg.projection_n - done)
-        u, v, p = emulate_launch(cfg, flags, u, v, p, tw, th, K, sw, R)
+        u, v, p = emulate_launch(cfg, flags, u, v, p, tw, th, K, sw, R, mask)
         done += sw
     return u[::-1], v[::-1], p[::-1]
 
 
+@pytest.mark.parametrize("mask", [True, False])
 @pytest.mark.parametrize("W,H,n,K,R,pressure", [
     (150, 40, 5, 2, 4, 0),     # two tiles across, several down, n not a multiple of k
     (130, 70, 4, 4, 8, 1),     # pressure on, k = 4
     (60, 25, 3, 3, 4, 0),      # grid smaller than one tile row
+    (250, 90, 11, 5, 8, 0),    # the default shape, tiles on both edges
 ])
-def test_regs_emulation_matches_oracle(W, H, n, K, R, pressure):
+def test_regs_emulation_matches_oracle(W, H, n, K, R, pressure, mask):
     cfg = make_config(W, H, projection_n=n, enable_pressure=pressure,
                       obstacle_center_x=W // 2 + 1, obstacle_center_y=H // 2 - 2,
                       obstacle_radius=min(6.5, H / 4))
     o = OracleFluid(cfg)
     o.set_state(u=noise((H, W), 31, 0.5), v=noise((H, W), 32, 0.5),
                 p=noise((H, W), 33, 0.5) if pressure else None)
-    u, v, p = emulate_projection(cfg, o, K, R)
+    u, v, p = emulate_projection(cfg, o, K, R, mask)
     o.projection()
     assert np.array_equal(u.view(np.uint32), o.u.view(np.uint32))
     assert np.array_equal(v.view(np.uint32), o.v.view(np.uint32))
