@@ -28,6 +28,14 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+
+# HBM bytes per projection launch measured with rocprofv3 FETCH_SIZE and
+# WRITE_SIZE passes (scaled by the same counters on a copy of known size),
+# per (kernel, tile width, tile height, sweeps per launch, W, H); summary and
+# commands in profiles/step_1920x1080_n50_fp32.md.
+MEASURED_LAUNCH_TRAFFIC = {
+    (2, 108, 44, 5, 1920, 1080): 58_889_692,
+}
 METRIC = "Gcell-steps/s at 1920x1080 n=50; % HBM roofline; p50/p99 batched step latency"
 
 
@@ -284,7 +292,9 @@ def main() -> int:
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": None,
+        "traffic": MEASURED_LAUNCH_TRAFFIC.get(
+            (tuning["projection_kernel"], tuning["tile_width"], tuning["tile_height"],
+             tuning["sweeps_per_launch"], W, H)),
         "kernel": fl.projection_kernel_name,
         "avg_launch_us": round(per_launch_ms * 1e3, 2),
         "alg_bytes_per_launch": int(alg_bytes_per_launch),
