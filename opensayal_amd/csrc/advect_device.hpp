@@ -108,10 +108,7 @@ __device__ __attribute__((always_inline)) inline void advect_velocity_tile(const
   auto vval = [&](int i, int j) -> float {
     return R.in(i, j) ? sv[R.at(i, j)] : v_extrapolated(v, g, i, j);
   };
-  // staged-only accessors (no bounds test) for cells known to be staged
-  auto svalid = [&](int i, int j) -> bool { return (sf[R.at(i, j)] & CF_FLUID) != 0; };
-  auto suval = [&](int i, int j) -> float { return su[R.at(i, j)]; };
-  auto svval = [&](int i, int j) -> float { return sv[R.at(i, j)]; };
+  auto fluid = [&](int l) -> bool { return (sf[l] & CF_FLUID) != 0; };
   const int cs = sp.cs;
   const float dt = sp.d_t;
   const int i = bx * kAdvTX + (tid & 63);
@@ -128,11 +125,12 @@ __device__ __attribute__((always_inline)) inline void advect_velocity_tile(const
     const float pux = (float)(CS1 ? i : i * cs);
     const float puy = CS1 ? (float)j + 0.5f : (float)(((double)j + 0.5) * (double)cs);
     float avg_v = sv[l];
-    {
-      const bool a = svalid(i - 1, j + 1), b = svalid(i, j + 1), c = svalid(i - 1, j);
-      avg_v = add_if(avg_v, a, svval(i - 1, j + 1));
-      avg_v = add_if(avg_v, b, svval(i, j + 1));
-      avg_v = add_if(avg_v, c, svval(i - 1, j));
+    {  // (i-1, j+1), (i, j+1), (i-1, j)
+      const int la = l - 1 + kAdvRX, lb = l + kAdvRX, lc = l - 1;
+      const bool a = fluid(la), b = fluid(lb), c = fluid(lc);
+      avg_v = add_if(avg_v, a, sv[la]);
+      avg_v = add_if(avg_v, b, sv[lb]);
+      avg_v = add_if(avg_v, c, sv[lc]);
       avg_v = div_count_bf(avg_v, 1 + (int)a + (int)b + (int)c);
     }
     const float xu = pux - su[l] * dt;
@@ -141,11 +139,12 @@ __device__ __attribute__((always_inline)) inline void advect_velocity_tile(const
     const float pvx = CS1 ? (float)i + 0.5f : (float)(((double)i + 0.5) * (double)cs);
     const float pvy = (float)(CS1 ? j : j * cs);
     float avg_u = su[l];
-    {
-      const bool a = svalid(i + 1, j), b = svalid(i, j - 1), c = svalid(i + 1, j - 1);
-      avg_u = add_if(avg_u, a, suval(i + 1, j));
-      avg_u = add_if(avg_u, b, suval(i, j - 1));
-      avg_u = add_if(avg_u, c, suval(i + 1, j - 1));
+    {  // (i+1, j), (i, j-1), (i+1, j-1)
+      const int la = l + 1, lb = l - kAdvRX, lc = l + 1 - kAdvRX;
+      const bool a = fluid(la), b = fluid(lb), c = fluid(lc);
+      avg_u = add_if(avg_u, a, su[la]);
+      avg_u = add_if(avg_u, b, su[lb]);
+      avg_u = add_if(avg_u, c, su[lc]);
       avg_u = div_count_bf(avg_u, 1 + (int)a + (int)b + (int)c);
     }
     const float xv = pvx - avg_u * dt;
@@ -154,8 +153,8 @@ __device__ __attribute__((always_inline)) inline void advect_velocity_tile(const
     const bool staged = R.inner(sample_cell<CS1>(xu, cs), sample_cell<CS1>(yu, cs)) &&
                         R.inner(sample_cell<CS1>(xv, cs), sample_cell<CS1>(yv, cs));
     if (__all(staged)) {
-      nu = sample_u_bf<CS1>(cs, xu, yu, svalid, suval);
-      nv = sample_v_bf<CS1>(cs, xv, yv, svalid, svval);
+      nu = lds_sample_u<CS1, kAdvRX>(cs, xu, yu, su, sf, R.x0, R.y0);
+      nv = lds_sample_v<CS1, kAdvRX>(cs, xv, yv, sv, sf, R.x0, R.y0);
     } else {
       nu = sample_u<CS1>(cs, xu, yu, valid, uval);
       nv = sample_v<CS1>(cs, xv, yv, valid, vval);
@@ -202,10 +201,7 @@ __device__ __attribute__((always_inline)) inline void advect_smoke_tile(const St
   auto sval = [&](int i, int j) -> float {
     return R.in(i, j) ? ss[R.at(i, j)] : rnd<T>(force_smoke(sp, i, j, ldf(smoke, g.at(i, j))));
   };
-  auto svalid = [&](int i, int j) -> bool { return (sf[R.at(i, j)] & CF_FLUID) != 0; };
-  auto suval = [&](int i, int j) -> float { return su[R.at(i, j)]; };
-  auto svval = [&](int i, int j) -> float { return sv[R.at(i, j)]; };
-  auto ssval = [&](int i, int j) -> float { return ss[R.at(i, j)]; };
+
   const int cs = sp.cs;
   const int i = bx * kAdvTX + (tid & 63);
 #pragma unroll
@@ -215,13 +211,13 @@ __device__ __attribute__((always_inline)) inline void advect_smoke_tile(const St
     const float cx = centre<CS1>(i, cs), cy = centre<CS1>(j, cs);
     // the velocity at a cell centre samples cells within one of (i, j),
     // which are staged
-    const float vx = sample_u_bf<CS1>(cs, cx, cy, svalid, suval);
-    const float vy = sample_v_bf<CS1>(cs, cx, cy, svalid, svval);
+    const float vx = lds_sample_u<CS1, kAdvRX>(cs, cx, cy, su, sf, R.x0, R.y0);
+    const float vy = lds_sample_v<CS1, kAdvRX>(cs, cx, cy, sv, sf, R.x0, R.y0);
     const float x = cx - vx * sp.d_t;
     const float y = cy - vy * sp.d_t;
     float s;
     if (__all(R.inner(sample_cell<CS1>(x, cs), sample_cell<CS1>(y, cs))))
-      s = sample_smoke_bf<CS1>(cs, x, y, svalid, ssval);
+      s = lds_sample_smoke<CS1, kAdvRX>(cs, x, y, ss, sf, R.x0, R.y0);
     else
       s = sample_smoke<CS1>(cs, x, y, valid, sval);
     s = rnd<T>(s);  // the advection's store, before the decay reads it back

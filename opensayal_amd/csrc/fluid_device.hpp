@@ -68,6 +68,17 @@ template <typename T> __device__ inline float ldf(const T* p, long k) { return S
 template <typename T> __device__ inline float rnd(float x) { return Store<T>::ld(Store<T>::st(x)); }
 template <typename T> __device__ inline void stf(T* p, long k, float x) { p[k] = Store<T>::st(x); }
 
+// Workgroup b of a launch of n runs on XCD b % 8 (workgroups are dispatched
+// round-robin over the 8 XCDs).  The tile it takes: XCD x gets a contiguous
+// run of the row-major tile order, so neighbouring tiles, whose halos
+// overlap, share that XCD's L2.  A bijection of [0, n) for every n.
+constexpr int kXcds = 8;
+__device__ inline int xcd_tile(int b, int n) {
+  const int x = b % kXcds, k = b / kXcds;
+  const int q = n / kXcds, r = n % kXcds;
+  return x * q + (x < r ? x : r) + k;
+}
+
 // Grid geometry and the flag array, passed by value to every kernel.
 struct GridView {
   int W, H, P, cs;
@@ -283,6 +294,64 @@ __device__ inline float sample_smoke(int cs, float x, float y, const Valid& vali
   return acc;
 }
 
+// The branch-free samplers over a row-major LDS stage of `RX` columns whose
+// cell (0, 0) is grid cell (x0, y0): one index per sample, neighbours at
+// constant offsets.  `a` holds the sampled field, `f` the cell flags.  Same
+// terms, order and results as sample_u_bf / sample_v_bf / sample_smoke_bf.
+template <bool CS1, int RX>
+__device__ inline float lds_sample_u(int cs, float x, float y, const float* a,
+                                     const uint8_t* f, int x0, int y0) {
+  const int i = to_index(div_cs<CS1>(x, cs));
+  const int j = to_index(div_cs<CS1>(y, cs));
+  const float in_x = x - (float)(CS1 ? i : i * cs);
+  const float in_y = y - (float)(CS1 ? j : j * cs);
+  const double h = cs / 2.0;
+  const bool low = CS1 ? (in_y <= 0.5f) : ((double)in_y <= h);
+  const float d_lo = CS1 ? 0.5f - in_y : (float)(h - (double)in_y);
+  const float d_hi = CS1 ? in_y - 0.5f : (float)((double)in_y - h);
+  const float w_x = 1.0f - div_cs<CS1>(in_x, cs);
+  const float w_y = 1.0f - div_cs<CS1>(low ? d_lo : d_hi, cs);
+  const int b = (j - y0) * RX + (i - x0);
+  const int bo = b + (low ? -RX : RX);
+  const float tA = (w_y * w_x) * a[b];
+  const float tB = (w_y * (1.0f - w_x)) * a[b + 1];
+  const float tC = ((1.0f - w_y) * w_x) * a[bo];
+  const float tD = ((1.0f - w_y) * (1.0f - w_x)) * a[bo + 1];
+  const bool vB = f[b + 1] & CF_FLUID, vC = f[bo] & CF_FLUID, vD = f[bo + 1] & CF_FLUID;
+  float acc = 0.0f + tA;
+  acc = low ? add_if(acc, vB, tB) : add_if(acc, vC, tC);
+  acc = low ? add_if(acc, vC, tC) : add_if(acc, vB, tB);
+  acc = add_if(acc, vD, tD);
+  return (f[b] & CF_FLUID) ? acc : 0.0f;
+}
+
+template <bool CS1, int RX>
+__device__ inline float lds_sample_v(int cs, float x, float y, const float* a,
+                                     const uint8_t* f, int x0, int y0) {
+  const int i = to_index(div_cs<CS1>(x, cs));
+  const int j = to_index(div_cs<CS1>(y, cs));
+  const float in_x = x - (float)(CS1 ? i : i * cs);
+  const float in_y = y - (float)(CS1 ? j : j * cs);
+  const double h = cs / 2.0;
+  const bool left = CS1 ? (in_x < 0.5f) : ((double)in_x < h);
+  const float d_l = CS1 ? 0.5f - in_x : (float)(h - (double)in_x);
+  const float d_r = CS1 ? in_x - 0.5f : (float)((double)in_x - h);
+  const float w_x = 1.0f - div_cs<CS1>(left ? d_l : d_r, cs);
+  const float w_y = 1.0f - div_cs<CS1>(in_y, cs);
+  const int b = (j - y0) * RX + (i - x0);
+  const int bo = b + (left ? -1 : 1);
+  const float tA = (w_y * w_x) * a[b];
+  const float tB = (w_y * (1.0f - w_x)) * a[bo];
+  const float tC = ((1.0f - w_y) * (1.0f - w_x)) * a[bo + RX];
+  const float tD = ((1.0f - w_y) * w_x) * a[b + RX];
+  const bool vB = f[bo] & CF_FLUID, vC = f[bo + RX] & CF_FLUID, vD = f[b + RX] & CF_FLUID;
+  float acc = 0.0f + tA;
+  acc = left ? add_if(acc, vB, tB) : add_if(acc, vD, tD);
+  acc = add_if(acc, vC, tC);
+  acc = left ? add_if(acc, vD, tD) : add_if(acc, vB, tB);
+  return (f[b] & CF_FLUID) ? acc : 0.0f;
+}
+
 template <bool CS1, class Valid, class Val>
 __device__ inline float sample_smoke_bf(int cs, float x, float y, const Valid& valid,
                                         const Val& val) {
@@ -307,6 +376,36 @@ __device__ inline float sample_smoke_bf(int cs, float x, float y, const Valid& v
   acc = add_if(acc, valid(i2, j), (inv2 / sum_inv) * val(i2, j));
   acc = add_if(acc, valid(i, j3), (inv3 / sum_inv) * val(i, j3));
   acc = add_if(acc, valid(i2, j3), (inv4 / sum_inv) * val(i2, j3));
+  return acc;
+}
+
+template <bool CS1, int RX>
+__device__ inline float lds_sample_smoke(int cs, float x, float y, const float* a,
+                                         const uint8_t* f, int x0, int y0) {
+  const int i = to_index(div_cs<CS1>(x, cs));
+  const int j = to_index(div_cs<CS1>(y, cs));
+  const float in_x = x - (float)(CS1 ? i : i * cs);
+  const float in_y = y - (float)(CS1 ? j : j * cs);
+  const double h = cs / 2.0;
+  const bool left = CS1 ? (in_x < 0.5f) : ((double)in_x < h);
+  const bool low = CS1 ? (in_y < 0.5f) : ((double)in_y < h);
+  const int i2 = left ? i - 1 : i + 1;
+  const int j3 = low ? j - 1 : j + 1;
+  const float x1 = centre<CS1>(i, cs), y1 = centre<CS1>(j, cs);
+  const float x2 = centre<CS1>(i2, cs), y3 = centre<CS1>(j3, cs);
+  const float inv1 = inv_distance(x - x1, y - y1);
+  const float inv2 = inv_distance(x - x2, y - y1);
+  const float inv3 = inv_distance(x - x1, y - y3);
+  const float inv4 = inv_distance(x - x2, y - y3);
+  const float sum_inv = ((inv1 + inv2) + inv3) + inv4;
+  const int b1 = (j - y0) * RX + (i - x0);
+  const int b2 = b1 + (left ? -1 : 1);
+  const int dj = low ? -RX : RX;
+  float acc = 0.0f;
+  acc = add_if(acc, f[b1] & CF_FLUID, (inv1 / sum_inv) * a[b1]);
+  acc = add_if(acc, f[b2] & CF_FLUID, (inv2 / sum_inv) * a[b2]);
+  acc = add_if(acc, f[b1 + dj] & CF_FLUID, (inv3 / sum_inv) * a[b1 + dj]);
+  acc = add_if(acc, f[b2 + dj] & CF_FLUID, (inv4 / sum_inv) * a[b2 + dj]);
   return acc;
 }
 

@@ -101,7 +101,8 @@ __global__ void __launch_bounds__(kTileThreads)
   const int q = cb * 64 + lane;            // this lane's cell pair
   const bool lane_on = q < pw;
 
-  const int tx = blockIdx.x % a.ntx, ty = blockIdx.x / a.ntx;
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int tx = tile % a.ntx, ty = tile / a.ntx;
   const int x0 = tx * a.tw, y0 = ty * a.th;
   const int gx0 = x0 - 2 * K, gy0 = y0 - 2 * K;  // region origin; gx0 is even
   const int W = sp.W, H = sp.H, P = sp.P;
@@ -250,7 +251,8 @@ __global__ void __launch_bounds__(64 * kRegWaves)
                    const T* __restrict__ v_in, T* __restrict__ u_out, T* __restrict__ v_out,
                    float* __restrict__ p, RegArgs a) {
   __shared__ RegShared sh;
-  project_regs_tile<T, PRESSURE, R>(sp, flags, u_in, v_in, u_out, v_out, p, a, blockIdx.x, sh);
+  project_regs_tile<T, PRESSURE, R>(sp, flags, u_in, v_in, u_out, v_out, p, a,
+                                    xcd_tile(blockIdx.x, gridDim.x), sh);
 }
 
 template <typename T, bool PRESSURE>

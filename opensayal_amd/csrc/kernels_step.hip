@@ -142,22 +142,26 @@ __global__ void __launch_bounds__(256) k_minmax_final(const float* part, int npa
 // ---- P6-P8: the advections over 64 x 16 tiles, 256 threads ---------------
 constexpr int kAdvTY = 16, kAdvNY = 4;
 
+// 1-D grids of ntx x nty tiles in the XCD-aware order (xcd_tile).
 template <typename T, bool CS1>
 __global__ void __launch_bounds__(64 * kAdvNY)
     k_advect_velocity(StepParams sp, GridView g, const T* __restrict__ u,
-                      const T* __restrict__ v, T* __restrict__ u_out, T* __restrict__ v_out) {
+                      const T* __restrict__ v, T* __restrict__ u_out, T* __restrict__ v_out,
+                      int ntx) {
   __shared__ AdvShared<kAdvTY> sh;
-  advect_velocity_tile<T, CS1, kAdvTY, kAdvNY>(sp, g, u, v, u_out, v_out, blockIdx.x, blockIdx.y,
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  advect_velocity_tile<T, CS1, kAdvTY, kAdvNY>(sp, g, u, v, u_out, v_out, tile % ntx, tile / ntx,
                                                sh);
 }
 
 template <typename T, bool CS1>
 __global__ void __launch_bounds__(64 * kAdvNY)
     k_advect_smoke(StepParams sp, GridView g, const T* __restrict__ u, const T* __restrict__ v,
-                   const T* __restrict__ smoke, T* __restrict__ smoke_out) {
+                   const T* __restrict__ smoke, T* __restrict__ smoke_out, int ntx) {
   __shared__ AdvShared<kAdvTY> sh;
-  advect_smoke_tile<T, CS1, kAdvTY, kAdvNY>(sp, g, u, v, smoke, smoke_out, blockIdx.x, blockIdx.y,
-                                            sh);
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  advect_smoke_tile<T, CS1, kAdvTY, kAdvNY>(sp, g, u, v, smoke, smoke_out, tile % ntx,
+                                            tile / ntx, sh);
 }
 
 // ---- divergence norms (Fluid::get_divergence, src/fluid.cu:192-201) -------
@@ -240,16 +244,17 @@ __global__ void k_layout(int mode, int W, int H, int P, T* dev, float* pdev, con
 template <typename T, bool CS1>
 void advect_t(const StepParams& sp, const GridView& g, const FieldPtrs& f, int cur, int* uv,
               int* sm, hipStream_t s) {
-  const dim3 grid((sp.W + kAdvTX - 1) / kAdvTX, (sp.H + kAdvTY - 1) / kAdvTY), block(64 * kAdvNY);
+  const int ntx = (sp.W + kAdvTX - 1) / kAdvTX, nty = (sp.H + kAdvTY - 1) / kAdvTY;
+  const dim3 grid(ntx * nty), block(64 * kAdvNY);
   const int nxt = 1 - cur;
   T* u2 = (T*)f.u[nxt];
   T* v2 = (T*)f.v[nxt];
   hipLaunchKernelGGL((k_advect_velocity<T, CS1>), grid, block, 0, s, sp, g, (const T*)f.u[cur],
-                     (const T*)f.v[cur], u2, v2);
+                     (const T*)f.v[cur], u2, v2, ntx);
   *uv = nxt;
   if (sp.smoke_on) {
     hipLaunchKernelGGL((k_advect_smoke<T, CS1>), grid, block, 0, s, sp, g, (const T*)u2,
-                       (const T*)v2, (const T*)f.smoke[*sm], (T*)f.smoke[1 - *sm]);
+                       (const T*)v2, (const T*)f.smoke[*sm], (T*)f.smoke[1 - *sm], ntx);
     *sm = 1 - *sm;
   }
 }
