@@ -34,7 +34,7 @@ HBM_PEAK_GBS = 8000.0
 # per (kernel, tile width, tile height, sweeps per launch, W, H); summary and
 # commands in profiles/step_1920x1080_n50_fp32.md.
 MEASURED_LAUNCH_TRAFFIC = {
-    (2, 108, 44, 5, 1920, 1080): 58_889_692,
+    (2, 108, 44, 5, 1920, 1080): 38_170_739,
 }
 METRIC = "Gcell-steps/s at 1920x1080 n=50; % HBM roofline; p50/p99 batched step latency"
 
@@ -270,21 +270,18 @@ def main() -> int:
     ms_per_step = gpu_ms / args.steps
     value = W * H * args.steps / (gpu_ms / 1e3) / 1e9
 
-    # dominant kernel: projection launches timed one call at a time with
-    # events on the stream they run on
+    # dominant kernel: kernel_reps projection launches of sweeps_per_launch
+    # sweeps back to back, timed with events on the stream they run on
     spl = fl.sweeps_per_launch
-    durations = []
-    for _ in range(args.kernel_reps):
-        a, b = Event(), Event()
-        a.record(stream)
-        fl.project(sweeps=spl, stream=stream, sync=False)
-        b.record(stream)
-        durations.append((a, b))
+    fl.project(sweeps=spl, stream=stream, sync=False)
+    a, b = Event(), Event()
+    a.record(stream)
+    fl.project(sweeps=spl * args.kernel_reps, stream=stream, sync=False)
+    b.record(stream)
     stream.synchronize()
-    per_call_ms = sum(a.elapsed_ms(b) for a, b in durations) / len(durations)
-    launches = fl.launches_per_call(spl)
-    per_launch_ms = per_call_ms / launches
-    alg_bytes_per_launch = 17 * spl * W * H / launches
+    launches = fl.launches_per_call(spl * args.kernel_reps)
+    per_launch_ms = a.elapsed_ms(b) / launches
+    alg_bytes_per_launch = 17 * spl * args.kernel_reps * W * H / launches
     achieved = alg_bytes_per_launch / (per_launch_ms / 1e3) / 1e9
     roofline = {
         "bound": "hbm",
