@@ -53,11 +53,13 @@ __device__ inline int sample_cell(float x, int cs) {
 template <typename T, int TY, int NY>
 __device__ __attribute__((always_inline)) inline void forces_tile(const StepParams& sp, int do_uv, int do_smoke, T* u, T* v,
                                    T* smoke, float* p, int bx, int by) {
-  const int i = bx * kAdvTX + (int)(threadIdx.x & 63);
+  int tid = (int)threadIdx.x;
+  asm volatile("" : "+v"(tid));
+  const int i = bx * kAdvTX + (tid & 63);
   if (i >= sp.W) return;
 #pragma unroll
   for (int m = 0; m < TY / NY; m++) {
-    const int j = by * TY + (int)(threadIdx.x >> 6) + NY * m;
+    const int j = by * TY + (tid >> 6) + NY * m;
     if (j >= sp.H) continue;
     const long k = (long)j * sp.P + i;
     if (do_uv) {
@@ -89,7 +91,10 @@ __device__ __attribute__((always_inline)) inline void advect_velocity_tile(const
   float* sv = sh.sv;
   uint8_t* sf = sh.sf;
   const AdvRegion<TY> R{bx * kAdvTX - kAdvHalo, by * TY - kAdvHalo};
-  const int tid = (int)threadIdx.x;
+  // the thread index is re-read per tile (not hoisted into a register that
+  // stays live across a persistent kernel's loop)
+  int tid = (int)threadIdx.x;
+  asm volatile("" : "+v"(tid));
   for (int q = tid; q < S::N; q += 64 * NY) {
     const int rr = q / kAdvRX, cc = q - rr * kAdvRX;
     const int i = R.x0 + cc, j = R.y0 + rr;
@@ -181,7 +186,10 @@ __device__ __attribute__((always_inline)) inline void advect_smoke_tile(const St
   float* ss = sh.ss;
   uint8_t* sf = sh.sf;
   const AdvRegion<TY> R{bx * kAdvTX - kAdvHalo, by * TY - kAdvHalo};
-  const int tid = (int)threadIdx.x;
+  // the thread index is re-read per tile (not hoisted into a register that
+  // stays live across a persistent kernel's loop)
+  int tid = (int)threadIdx.x;
+  asm volatile("" : "+v"(tid));
   for (int q = tid; q < S::N; q += 64 * NY) {
     const int rr = q / kAdvRX, cc = q - rr * kAdvRX;
     const int i = R.x0 + cc, j = R.y0 + rr;

@@ -38,6 +38,7 @@ struct BatchSim {
   void* smoke[2];
   float* p;
   const uint8_t* flags;
+  const uint8_t* gates;
   int uv0, sm0;
   int flip_uv, flip_sm;
 };

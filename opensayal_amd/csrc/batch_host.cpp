@@ -148,6 +148,7 @@ int rebuild(osy_batch* b) {
       }
       s.p = fl->f.p;
       s.flags = fl->f.flags;
+      s.gates = fl->f.gates;
       s.uv0 = fl->uv;
       s.sm0 = fl->sm;
       const int ns = sp.n > 0 ? ceil_div(sp.n, k) : 0;

@@ -74,10 +74,10 @@ size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
 // Sizes of the parts of a handle's device allocation: six fields of the
 // storage type (u, v, smoke, double-buffered), pressure, a reference-order
-// staging buffer, the cell flags, and reduction scratch.
+// staging buffer, the cell flags and gate bytes, and reduction scratch.
 struct BlockLayout {
   size_t fld, f32, ref, flg, red, dred;
-  size_t total() const { return 6 * fld + f32 + ref + flg + red + dred; }
+  size_t total() const { return 6 * fld + f32 + ref + 2 * flg + red + dred; }
 };
 
 BlockLayout block_layout(int W, int H, int storage) {
@@ -153,6 +153,7 @@ int create_impl(const osy_config* cfg, osy_fluid** out) {
   fl->f.p = (float*)b; b += f32;
   fl->staging = (float*)b; b += ref;
   fl->f.flags = (uint8_t*)b; b += flg;
+  fl->f.gates = (uint8_t*)b; b += flg;
   fl->f.scratch = (float*)b; b += red;
   fl->f.dscratch = (double*)b; b += dred;
   fl->f.prange = fl->f.scratch + 2 * kReduceBlocks;
@@ -169,7 +170,7 @@ int create_impl(const osy_config* cfg, osy_fluid** out) {
     m.radius = (double)cfg->obstacle_radius;
     m.pipe_length = cfg->pipe_length;
     m.pipe_height = cfg->pipe_height;
-    launch_build_flags(m, fl->f.flags, fl->stream);
+    launch_build_flags(m, fl->f.flags, fl->f.gates, fl->stream);
     e = hipGetLastError();
   }
   if (e == hipSuccess) e = hipStreamSynchronize(fl->stream);

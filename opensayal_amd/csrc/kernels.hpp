@@ -49,6 +49,7 @@ struct FieldPtrs {
   void* smoke[2];
   float* p;
   uint8_t* flags;
+  uint8_t* gates;   // projection gate byte per cell (static: built with the flags)
   float* scratch;   // reductions: partial results
   float* prange;    // [min, max] of pressure
   double* dscratch; // divergence reduction partials
@@ -114,7 +115,8 @@ struct ProjTuning {
 // Number of float slots `scratch` must hold, and doubles for `dscratch`.
 constexpr int kReduceBlocks = 512;
 
-void launch_build_flags(const MaskParams& m, uint8_t* flags, hipStream_t s);
+// Cell flags, then the projection gate bytes derived from them.
+void launch_build_flags(const MaskParams& m, uint8_t* flags, uint8_t* gates, hipStream_t s);
 
 // One full step (src/fluid.cu:770-795).  `uv`/`sm` give the buffer index
 // currently holding u,v / smoke and are advanced to the index holding the

@@ -247,11 +247,11 @@ __global__ void __launch_bounds__(kTileThreads)
 
 template <typename T, bool PRESSURE, int R>
 __global__ void __launch_bounds__(64 * kRegWaves)
-    k_project_regs(StepParams sp, const uint8_t* __restrict__ flags, const T* __restrict__ u_in,
+    k_project_regs(StepParams sp, const uint8_t* __restrict__ gates, const T* __restrict__ u_in,
                    const T* __restrict__ v_in, T* __restrict__ u_out, T* __restrict__ v_out,
                    float* __restrict__ p, RegArgs a) {
   __shared__ RegShared sh;
-  project_regs_tile<T, PRESSURE, R>(sp, flags, u_in, v_in, u_out, v_out, p, a,
+  project_regs_tile<T, PRESSURE, R>(sp, gates, u_in, v_in, u_out, v_out, p, a,
                                     xcd_tile(blockIdx.x, gridDim.x), sh);
 }
 
@@ -293,7 +293,7 @@ void launch_regs_r(const StepParams& sp, const GridView& g, const FieldPtrs& f, 
     a.forces = (done == 0 && sp.forces_in_projection) ? 1 : 0;
     const int nxt = 1 - *cur;
     hipLaunchKernelGGL((k_project_regs<T, PRESSURE, R>), dim3(a.ntx * nty), dim3(64 * kRegWaves),
-                       0, s, sp, g.flags, (const T*)f.u[*cur], (const T*)f.v[*cur], (T*)f.u[nxt],
+                       0, s, sp, f.gates, (const T*)f.u[*cur], (const T*)f.v[*cur], (T*)f.u[nxt],
                        (T*)f.v[nxt], f.p, a);
     *cur = nxt;
   }

@@ -16,6 +16,7 @@
 #include "kernels.hpp"
 #include "batch.hpp"
 #include "projection.hpp"
+#include "projection_device.hpp"
 #include "step_device.hpp"
 
 #pragma clang fp contract(off)
@@ -69,6 +70,24 @@ __global__ void k_build_flags(MaskParams m, uint8_t* flags) {
     if (b) f |= CF_FV;
   }
   flags[(long)j * m.P + i] = f;
+}
+
+// The projection's gate byte of every cell (gate_byte: its face gates,
+// projected bit and quotient class), from its flags and those of its right
+// and top neighbours; 0 outside the grid and in the row padding.
+__global__ void k_build_gates(MaskParams m, const uint8_t* flags, uint8_t* gates) {
+  const int i = blockIdx.x * kBX + threadIdx.x;
+  const int j = blockIdx.y * kBY + threadIdx.y;
+  if (i >= m.P || j >= m.H) return;
+  const long k = (long)j * m.P + i;
+  if (i >= m.W) {
+    gates[k] = 0;
+    return;
+  }
+  const uint32_t f = flags[k];
+  const uint32_t fr = i + 1 < m.W ? flags[k + 1] : 0u;
+  const uint32_t ft = j + 1 < m.H ? flags[k + m.P] : 0u;
+  gates[k] = (uint8_t)gate_byte(f, fr, ft);
 }
 
 // ---- P1: forces over 64 x 4 tiles (forces_tile) -------------------------
@@ -315,8 +334,9 @@ void layout_t(int field, bool to_ref, const StepParams& sp, const FieldPtrs& f, 
 
 }  // namespace
 
-void launch_build_flags(const MaskParams& m, uint8_t* flags, hipStream_t s) {
+void launch_build_flags(const MaskParams& m, uint8_t* flags, uint8_t* gates, hipStream_t s) {
   hipLaunchKernelGGL(k_build_flags, cell_grid(m.P, m.H), dim3(kBX, kBY), 0, s, m, flags);
+  hipLaunchKernelGGL(k_build_gates, cell_grid(m.P, m.H), dim3(kBX, kBY), 0, s, m, flags, gates);
 }
 
 void enqueue_step(int storage, const StepParams& sp, const FieldPtrs& f, int* uv, int* sm,

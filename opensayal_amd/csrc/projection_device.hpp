@@ -280,15 +280,17 @@ struct RegShared {
 // over the grid's ntx tiles per row), run by a workgroup of 64 * kRegWaves
 // threads.
 template <typename T, bool PRESSURE, int R>
-__device__ __attribute__((always_inline)) inline void project_regs_tile(const StepParams& sp, const uint8_t* __restrict__ flags,
+__device__ __attribute__((always_inline)) inline void project_regs_tile(const StepParams& sp, const uint8_t* __restrict__ gates,
                                          const T* __restrict__ u_in, const T* __restrict__ v_in,
                                          T* __restrict__ u_out, T* __restrict__ v_out,
                                          float* __restrict__ p, const RegArgs& a, int tile,
                                          RegShared& sh) {
   float2 (*vrow)[64] = sh.vrow;
   float (*dsend)[64] = sh.dsend;
-  const int lane = threadIdx.x & 63;
-  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  int tid = (int)threadIdx.x;
+  asm volatile("" : "+v"(tid));  // re-read per tile, not kept live across a persistent loop
+  const int lane = tid & 63;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int K = a.k;
   const int tx = tile % a.ntx, ty = tile / a.ntx;
   const int x0 = tx * a.tw, y0 = ty * a.th;
@@ -300,21 +302,6 @@ __device__ __attribute__((always_inline)) inline void project_regs_tile(const St
 
   RegTile<R> t;
   t.slow = 0;
-  // flags: rows rbase .. rbase+R (one above for the top faces), columns
-  // 2l, 2l+1 of this lane and 2l+2 (lane l+1's first column; lane 63 loads
-  // its own)
-  uint32_t fl0[R + 1], fl1[R + 1], fl2[R + 1];
-#pragma unroll
-  for (int rho = 0; rho <= R; rho++) {
-    const int gj = gy0 + rbase + rho;
-    const bool row_in = (unsigned)gj < (unsigned)H;
-    const long gk = (long)gj * P + gi;
-    fl0[rho] = (row_in && col0) ? flags[gk] : 0u;
-    fl1[rho] = (row_in && col1) ? flags[gk + 1] : 0u;
-    const uint32_t from_right = lane_from_right_u(fl0[rho]);
-    fl2[rho] = (lane == 63) ? ((row_in && (unsigned)(gi + 2) < (unsigned)W) ? flags[gk + 2] : 0u)
-                            : from_right;
-  }
 #pragma unroll
   for (int rho = 0; rho < R; rho++) {
     const int gj = gy0 + rbase + rho;
@@ -340,8 +327,13 @@ __device__ __attribute__((always_inline)) inline void project_regs_tile(const St
     } else {
       t.s0[rho] = t.s1[rho] = 0.0f;
     }
-    t.gw[rho] = gate_byte(fl0[rho], fl1[rho], fl0[rho + 1]) |
-                (gate_byte(fl1[rho], fl2[rho], fl1[rho + 1]) << 8);
+    // the gate bytes of cells 2l and 2l+1 (static, built with the flags)
+    if (row_in && col0 && col1) {
+      t.gw[rho] = *reinterpret_cast<const uint16_t*>(gates + gk);
+    } else {
+      t.gw[rho] = ((row_in && col0) ? (uint32_t)gates[gk] : 0u) |
+                  ((row_in && col1) ? (uint32_t)gates[gk + 1] << 8 : 0u);
+    }
     t.slow |= (__any((t.gw[rho] & G_SLOW) != 0) ? 1u : 0u) << (2 * rho);
     t.slow |= (__any((t.gw[rho] & (G_SLOW << 8)) != 0) ? 1u : 0u) << (2 * rho + 1);
   }
