@@ -113,9 +113,6 @@ __device__ inline float lane_from_right(float x) {  // lane l gets lane l+1's x 
 __device__ inline float lane_from_left(float x) {  // lane l gets lane l-1's x (0 for lane 0)
   return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x138, 0xf, 0xf, true));
 }
-__device__ inline uint32_t lane_from_right_u(uint32_t x) {
-  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x130, 0xf, 0xf, true);
-}
 
 // gate of d: d when the bit is set, else +0
 __device__ inline float gate_pos(float d, uint32_t g, int bit) {
