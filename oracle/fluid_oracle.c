@@ -226,5 +226,114 @@ int orc_build_masks(const osy_config* c, int32_t* is_solid, int32_t* total_s) {
 ORC_EXPORTS(float, _f32)
 ORC_EXPORTS(double, _f64)
 
+/* ---- frame export: update_fluid_pixels_kernel (src/graphics_handler.cu:
+ * 214-285) with hsv_to_rgb and map_rgba (src/helper.cu:3-47) -----------------
+ * float -> integer conversions follow the GPU's convert instructions: round
+ * toward zero, saturate to the destination range, NaN to 0 (the reference's
+ * static_cast is undefined outside the range). */
+static uint32_t orc_sat_u8(float x) {
+  if (!(x > 0.0f)) return 0; /* negatives, -0 and NaN */
+  if (x >= 255.0f) return 255;
+  return (uint32_t)x;
+}
+
+static void orc_hsv_to_rgb(float h, float s, float v, uint32_t* r, uint32_t* g, uint32_t* b) {
+  const float c = v * s;
+  const float x = c * (1.0f - fabsf(fmodf(h / 60.0f, 2.0f) - 1.0f));
+  const float m = v - c;
+  float r_, g_, b_;
+  if (h < 60.0f) {
+    r_ = c; g_ = x; b_ = 0.0f;
+  } else if (h < 120.0f) {
+    r_ = x; g_ = c; b_ = 0.0f;
+  } else if (h < 180.0f) {
+    r_ = 0.0f; g_ = c; b_ = x;
+  } else if (h < 240.0f) {
+    r_ = 0.0f; g_ = x; b_ = c;
+  } else if (h < 300.0f) {
+    r_ = x; g_ = 0.0f; b_ = c;
+  } else {
+    r_ = c; g_ = 0.0f; b_ = x;
+  }
+  *r = orc_sat_u8((r_ + m) * 255.0f);
+  *g = orc_sat_u8((g_ + m) * 255.0f);
+  *b = orc_sat_u8((b_ + m) * 255.0f);
+}
+
+static uint32_t orc_rgba(uint32_t r, uint32_t g, uint32_t b, uint32_t a) {
+  return r << 24 | g << 16 | b << 8 | a;
+}
+
+static float orc_clampf(float x, float lo, float hi) { return (x < lo) ? lo : (x > hi) ? hi : x; }
+
+/* One pixel; the image is in the reference's indx order (row 0 = top),
+ * which is the screen order of the reference's pixel buffer. */
+uint32_t orc_pixel(int solid, int pressure_on, int smoke_on, float smoke, float p, float pmin,
+                   float pmax) {
+  uint32_t r, g, b;
+  if (solid) return orc_rgba(80, 80, 80, 255);
+  if (pressure_on && smoke_on) {
+    float norm = 0.0f;
+    if (p < 0.0f && pmin != 0.0f)
+      norm = -p / pmin;
+    else if (pmax != 0.0f)
+      norm = p / pmax;
+    norm = orc_clampf(norm, -1.0f, 1.0f);
+    orc_hsv_to_rgb((1.0f - norm) * 120.0f, 1.0f, smoke, &r, &g, &b);
+    return orc_rgba(r, g, b, 255);
+  }
+  if (pressure_on) {
+    float norm = p < 0.0f ? -p / pmin : p / pmax;
+    norm = orc_clampf(norm, -1.0f, 1.0f);
+    orc_hsv_to_rgb((1.0f - norm) * 120.0f, 1.0f, 1.0f, &r, &g, &b);
+    return orc_rgba(r, g, b, 255);
+  }
+  if (smoke_on) {
+    const uint32_t c = (255u - orc_sat_u8(smoke * 255.0f)) & 255u;
+    return orc_rgba(255, c, c, 255);
+  }
+  return 0; /* the reference leaves such pixels unwritten */
+}
+
+void orc_render_rgba(const osy_config* c, const int32_t* solid, const float* smoke,
+                     const float* p, float pmin, float pmax, uint32_t* out) {
+  const long n = (long)c->width * c->height;
+  for (long k = 0; k < n; k++)
+    out[k] = orc_pixel(solid[k], c->enable_pressure, c->enable_smoke, smoke[k], p[k], pmin, pmax);
+}
+
+/* ---- path lines: Fluid::trace (src/fluid.cu:16-36) ----------------------
+ * From the centre of cell (i, j), `length` points: position += velocity *
+ * d_t with (GX, GY) at each position; each point stored as
+ * (round(x), H - 1 - round(y)), round half away from zero, converted like
+ * the GPU's convert instruction (saturate, NaN to 0). */
+static int32_t orc_sat_i32(float x) {
+  if (x != x) return 0;
+  if (x >= 2147483648.0f) return 2147483647;
+  if (x < -2147483648.0f) return (int32_t)0x80000000u;
+  return (int32_t)x;
+}
+
+void orc_trace(const osy_config* c, const int32_t* solid, const int32_t* ts, const float* u,
+               const float* v, const int32_t* seeds, int count, int length, float d_t,
+               int32_t* out) {
+  ORC_GRID(c, solid, ts);
+  for (int s = 0; s < count; s++) {
+    float x = orc_center_f32(seeds[2 * s], g.cs);
+    float y = orc_center_f32(seeds[2 * s + 1], g.cs);
+    for (int k = 0; k < length; k++) {
+      if (k > 0) {
+        const float vx = orc_gx_f32(&g, u, x, y);
+        const float vy = orc_gy_f32(&g, v, x, y);
+        x = x + vx * d_t;
+        y = y + vy * d_t;
+      }
+      int32_t* o = out + 2 * ((long)s * length + k);
+      o[0] = orc_sat_i32(roundf(x));
+      o[1] = g.H - 1 - orc_sat_i32(roundf(y));
+    }
+  }
+}
+
 /* Lets a loader check that it opened this library and not another. */
 int orc_abi_marker(void) { return 0x05a7a1; }
