@@ -150,6 +150,28 @@ def batch_leg(specs, steps: int, warmup: int, plain_steps: int = 3, autotune: bo
     }
 
 
+def view_leg(fl, reps: int) -> dict:
+    """What GraphicsHandler reads each frame, as the synchronous library
+    calls a viewer makes: the RGBA frame (kernel plus the W*H*4-byte copy to
+    host) and the path lines at the config defaults (distance 20, length 20).
+    Wall-clock per call; the kernels' own times are in the rocprofv3 trace."""
+    fl.render_rgba()
+    fl.traces(20, 20)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        fl.render_rgba()
+    frame_ms = (time.perf_counter() - t0) * 1e3 / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        fl.traces(20, 20)
+    traces_ms = (time.perf_counter() - t0) * 1e3 / reps
+    W, H = fl.width, fl.height
+    return {"calls": reps, "ms_per_frame": round(frame_ms, 4),
+            "frame_host_GBs": round(W * H * 4 / (frame_ms / 1e3) / 1e9, 2),
+            "ms_per_traces": round(traces_ms, 4),
+            "seeds": (W // 20) * (H // 20), "trace_length": 20}
+
+
 def sweep_shapes(fl, stream, steps: int = 40) -> None:
     """Per-step time for a grid of projection launch shapes (one JSON line each)."""
     from opensayal_amd import Event
@@ -194,6 +216,8 @@ def main() -> int:
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--kernel-reps", type=int, default=20)
+    ap.add_argument("--no-view", action="store_true", help="skip the frame export leg")
+    ap.add_argument("--view-reps", type=int, default=20)
     ap.add_argument("--requests", type=int, default=64, help="serving: number of requests")
     ap.add_argument("--rate", type=float, default=8.0, help="serving: arrivals per second")
     ap.add_argument("--budget-gb", type=float, default=200.0, help="serving: HBM state budget")
@@ -321,6 +345,8 @@ def main() -> int:
                           "frac": round(step_alg / HBM_PEAK_GBS, 4)},
         "wall_s": round(wall, 3),
     }
+    if not args.no_view:
+        out["frame_export"] = view_leg(fl, args.view_reps)
     fl.close()
     if not args.no_batch and args.workload == "full":
         # 256 uneven grids (128..1024 wide, n from {20, 50, 100}) and 256

@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define OSY_ABI_VERSION 3
+#define OSY_ABI_VERSION 4
 
 typedef enum osy_status {
   OSY_OK = 0,
@@ -219,6 +219,36 @@ int osy_fluid_pressure_range(osy_fluid* fluid, float* min_pressure,
 int osy_fluid_divergence_norm(osy_fluid* fluid, double* l2, double* linf);
 
 int osy_fluid_get_info(const osy_fluid* fluid, osy_fluid_info* info);
+
+/* ---- what the reference's GraphicsHandler reads from a Fluid each frame --- */
+
+/* The frame of update_fluid_pixels_kernel (src/graphics_handler.cu:258-285)
+ * and its copy to host (GraphicsHandler::update_fluid_pixels, :287-300): one
+ * RGBA8888 pixel per cell, r << 24 | g << 16 | b << 8 | a (map_rgba,
+ * src/helper.cu:45-47), in the reference's indx order (row 0 = the top row
+ * of the screen).  Solid cells are (80, 80, 80, 255); with pressure and smoke
+ * on the hue is (1 - norm) * 120 degrees, norm the pressure over the last
+ * update's min / max (hsv_to_rgb, src/helper.cu:3-43), value the smoke; with
+ * pressure only the value is 1; with smoke only (255, c, c, 255),
+ * c = 255 - 255 * smoke; with neither the pixel is 0.  Float -> byte
+ * conversions truncate and saturate, NaN giving 0.  `bytes` must equal
+ * W*H*4.  Synchronous. */
+int osy_fluid_render_rgba(osy_fluid* fluid, uint32_t* dst, size_t bytes);
+
+/* Path lines of update_traces_kernel (src/graphics_handler.cu:384-396) with
+ * Fluid::trace (src/fluid.cu:16-36) and the copies to host (:406-412): seeds
+ * at (ti * distance, tj * distance), ti < W / distance, tj < H / distance;
+ * seed (ti, tj) owns entries [o, o + length) of traces_x and traces_y with
+ * o = ((H/distance - 1 - tj) * (W/distance) + ti) * length (indx_traces,
+ * src/graphics_handler.cu:7-11).  Point k is the cell centre advanced k times
+ * by the general velocity times d_t, stored as (round(x), H - 1 - round(y)),
+ * round half away from zero, saturating, NaN giving 0.  A seed on a solid
+ * cell gets -1 in all its entries (the reference sets entry 0, which its
+ * drawing loop tests).  `bytes` is the size of each of traces_x and
+ * traces_y and must equal (W/distance) * (H/distance) * length * 4;
+ * distance >= 1, 1 <= length <= 65536.  Synchronous. */
+int osy_fluid_traces(osy_fluid* fluid, int32_t distance, int32_t length, float d_t,
+                     int32_t* traces_x, int32_t* traces_y, size_t bytes);
 
 /* Select the projection launch shape (OSY_E_INVALID if the shape does not
  * fit the kernel's limits).  Captured step graphs are rebuilt on next use. */

@@ -279,6 +279,34 @@ class Fluid:
         check(lib().osy_fluid_divergence_norm(self._h, ctypes.byref(l2), ctypes.byref(mx)))
         return l2.value, mx.value
 
+    # -- what GraphicsHandler reads each frame ------------------------------
+    def render_rgba(self) -> np.ndarray:
+        """The frame of update_fluid_pixels_kernel
+        (src/graphics_handler.cu:258-285): (H, W) uint32 RGBA8888 pixels
+        (r << 24 | g << 16 | b << 8 | a), row 0 the top of the screen."""
+        out = np.empty((self.height, self.width), dtype=np.uint32)
+        check(lib().osy_fluid_render_rgba(self._h, out.ctypes.data_as(ctypes.c_void_p),
+                                          out.nbytes))
+        return out
+
+    def traces(self, distance: int = 20, length: int = 20,
+               d_t: Optional[float] = None) -> tuple:
+        """Path lines of update_traces_kernel (src/graphics_handler.cu:384-396)
+        for seeds every ``distance`` cells (visual.path_line.distance / length
+        default to 20): (traces_x, traces_y), each an int32 array of shape
+        (H // distance, W // distance, length) in indx_traces order (row 0 =
+        the top row of seeds); a solid seed's entries are -1."""
+        if distance < 1:
+            raise ValueError("distance must be at least 1")
+        dt = self.config.d_t if d_t is None else d_t
+        shape = (self.height // distance, self.width // distance, int(length))
+        tx = np.empty(shape, dtype=np.int32)
+        ty = np.empty(shape, dtype=np.int32)
+        check(lib().osy_fluid_traces(self._h, int(distance), int(length), float(dt),
+                                     tx.ctypes.data_as(ctypes.c_void_p),
+                                     ty.ctypes.data_as(ctypes.c_void_p), tx.nbytes))
+        return tx, ty
+
 
 def device_bytes(config: Config) -> int:
     """Device bytes a Fluid of this configuration allocates."""

@@ -11,6 +11,8 @@ import os
 from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().parent / "libopensayal_amd.so"
+# the headless program (cli.cpp), linked against LIB_PATH by build.py
+CLI_PATH = Path(__file__).resolve().parent / "fluid-simulator"
 
 OSY_OK = 0
 OSY_E_INVALID = -1
@@ -118,6 +120,10 @@ PROTOTYPES = {
     "osy_fluid_pressure_range": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(_f32), ctypes.POINTER(_f32)]),
     "osy_fluid_divergence_norm": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
     "osy_fluid_get_info": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(OsyFluidInfo)]),
+    "osy_fluid_render_rgba": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]),
+    "osy_fluid_traces": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32,
+                                        ctypes.c_float, ctypes.c_void_p, ctypes.c_void_p,
+                                        ctypes.c_size_t]),
     "osy_fluid_set_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(OsyTuning)]),
     "osy_fluid_get_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(OsyTuning)]),
     "osy_fluid_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(OsyTuning),

@@ -19,7 +19,10 @@ from pathlib import Path
 PKG = Path(__file__).resolve().parent
 CSRC = PKG / "csrc"
 LIB = PKG / "libopensayal_amd.so"
+CLI = PKG / "fluid-simulator"
 OBJ = PKG / "_build"
+ROOT = PKG.parent
+CXX = os.environ.get("CXX", "g++")
 
 SOURCES = [
     "json_value.cpp",
@@ -29,6 +32,7 @@ SOURCES = [
     "kernels_step.hip",
     "kernels_project.hip",
     "kernels_batch.hip",
+    "kernels_view.hip",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -72,7 +76,25 @@ def build(verbose: bool = True) -> Path:
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB}", file=sys.stderr)
+    build_cli(verbose)
     return LIB
+
+
+def build_cli(verbose: bool = True) -> Path:
+    """Links the headless fluid-simulator program against the library; it
+    has no device code, so the host compiler builds it."""
+    tmp = CLI.with_suffix(".tmp")
+    cmd = [CXX, "-O2", "-std=c++17", "-Wall", "-Wextra", "-o", str(tmp),
+           str(CSRC / "cli.cpp"), str(CSRC / "json_value.cpp"),
+           f"-I{ROOT / 'include'}", f"-L{LIB.parent}", "-lopensayal_amd",
+           "-Wl,-rpath,$ORIGIN"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"link failed: {' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
+    os.replace(tmp, CLI)
+    if verbose:
+        print(f"built {CLI}", file=sys.stderr)
+    return CLI
 
 
 def clean() -> None:

@@ -14,25 +14,6 @@
 namespace osy {
 namespace {
 
-// ConfigParser::get_or (inc/config_parser.hpp:131-150): a direct hit on the
-// whole key wins; otherwise split at the first '.' and look the rest up in
-// the child object.  A conversion error on a direct hit propagates (at the
-// top level the reference dies with an uncaught exception; here the parse
-// fails with OSY_E_CONFIG); any error below the top level yields the default.
-template <typename T, typename Conv>
-T get_or_impl(const Json& j, const std::string& key, T def, Conv conv) {
-  if (j.contains(key)) return conv(j.at(key));
-  if (key.empty()) return def;
-  size_t dot = key.find('.');
-  if (dot == std::string::npos) return def;
-  std::string parent = key.substr(0, dot), child = key.substr(dot + 1);
-  try {
-    return get_or_impl(j.at(parent), child, def, conv);
-  } catch (...) {
-    return def;
-  }
-}
-
 int32_t as_int(const Json& v) {
   int64_t x = v.to_int64();
   if (x < INT32_MIN || x > INT32_MAX) throw JsonError("integer out of int32 range");
@@ -42,13 +23,13 @@ float as_float(const Json& v) { return (float)v.to_double(); }
 int32_t as_bool(const Json& v) { return v.to_bool() ? 1 : 0; }
 
 int32_t get_int(const Json& j, const char* key, int32_t def) {
-  return get_or_impl<int32_t>(j, key, def, as_int);
+  return json_get_or<int32_t>(j, key, def, as_int);
 }
 float get_float(const Json& j, const char* key, float def) {
-  return get_or_impl<float>(j, key, def, as_float);
+  return json_get_or<float>(j, key, def, as_float);
 }
 int32_t get_bool(const Json& j, const char* key, int32_t def) {
-  return get_or_impl<int32_t>(j, key, def, as_bool);
+  return json_get_or<int32_t>(j, key, def, as_bool);
 }
 
 }  // namespace
@@ -142,7 +123,7 @@ int config_parse_text(const std::string& text, osy_config* out) {
     c.viscosity = get_float(j, "fluid.viscosity", 0.001f);
     // this implementation's storage key (string), default f32
     c.storage = OSY_STORAGE_F32;
-    std::string storage = get_or_impl<std::string>(
+    std::string storage = json_get_or<std::string>(
         j, "sim.storage", std::string("f32"),
         [](const Json& v) { return v.str(); });
     if (storage == "f32") c.storage = OSY_STORAGE_F32;

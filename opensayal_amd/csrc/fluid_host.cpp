@@ -381,6 +381,57 @@ int osy_fluid_pressure_range(osy_fluid* fl, float* min_p, float* max_p) {
   return OSY_OK;
 }
 
+int osy_fluid_render_rgba(osy_fluid* fl, uint32_t* dst, size_t bytes) {
+  if (!fl || !dst) return set_error(OSY_E_INVALID, "null argument");
+  if (bytes != (size_t)fl->W * fl->H * 4)
+    return set_error(OSY_E_INVALID, "buffer must hold width*height 4-byte pixels");
+  if (int st = batch_wait(fl)) return st;
+  const StepParams sp = fl->params(fl->cfg.d_t, nullptr);
+  if (fl->cfg.enable_pressure && fl->batch) {  // a batch step does not reduce
+    enqueue_pressure_range(sp, fl->f, fl->stream);
+    OSY_HIP_CHECK(hipGetLastError());
+  }
+  enqueue_render(fl->storage, sp, fl->f, fl->sm, fl->cfg.enable_pressure != 0,
+                 fl->cfg.enable_smoke != 0, (uint32_t*)fl->staging, fl->stream);
+  OSY_HIP_CHECK(hipGetLastError());
+  OSY_HIP_CHECK(hipMemcpyAsync(dst, fl->staging, bytes, hipMemcpyDeviceToHost, fl->stream));
+  OSY_HIP_CHECK(hipStreamSynchronize(fl->stream));
+  return OSY_OK;
+}
+
+int osy_fluid_traces(osy_fluid* fl, int32_t distance, int32_t length, float d_t,
+                     int32_t* traces_x, int32_t* traces_y, size_t bytes) {
+  if (!fl || !traces_x || !traces_y) return set_error(OSY_E_INVALID, "null argument");
+  if (distance < 1) return set_error(OSY_E_INVALID, "distance must be at least 1");
+  if (length < 1 || length > 65536)
+    return set_error(OSY_E_INVALID, "length must be in [1, 65536]");
+  if (!std::isfinite(d_t)) return set_error(OSY_E_INVALID, "d_t must be finite");
+  const size_t seeds = (size_t)(fl->W / distance) * (size_t)(fl->H / distance);
+  const size_t each = seeds * (size_t)length * 4;
+  if (bytes != each)
+    return set_error(OSY_E_INVALID,
+                     "each buffer must hold (W/distance)*(H/distance)*length int32 entries");
+  if (each == 0) return OSY_OK;
+  if (int st = batch_wait(fl)) return st;
+  if (fl->trace_cap < 2 * each) {
+    OSY_HIP_CHECK(hipStreamSynchronize(fl->stream));
+    if (fl->trace_buf) OSY_HIP_CHECK(hipFree(fl->trace_buf));
+    fl->trace_buf = nullptr;
+    fl->trace_cap = 0;
+    OSY_HIP_CHECK(hipMalloc(&fl->trace_buf, 2 * each));
+    fl->trace_cap = 2 * each;
+  }
+  int32_t* dx = (int32_t*)fl->trace_buf;
+  int32_t* dy = dx + each / 4;
+  const StepParams sp = fl->params(d_t, nullptr);
+  enqueue_traces(fl->storage, sp, fl->f, fl->uv, distance, length, d_t, dx, dy, fl->stream);
+  OSY_HIP_CHECK(hipGetLastError());
+  OSY_HIP_CHECK(hipMemcpyAsync(traces_x, dx, each, hipMemcpyDeviceToHost, fl->stream));
+  OSY_HIP_CHECK(hipMemcpyAsync(traces_y, dy, each, hipMemcpyDeviceToHost, fl->stream));
+  OSY_HIP_CHECK(hipStreamSynchronize(fl->stream));
+  return OSY_OK;
+}
+
 int osy_fluid_divergence_norm(osy_fluid* fl, double* l2, double* linf) {
   if (!fl || !l2 || !linf) return set_error(OSY_E_INVALID, "null argument");
   if (int st = batch_wait(fl)) return st;

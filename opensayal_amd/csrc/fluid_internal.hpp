@@ -32,6 +32,8 @@ struct osy_fluid {
   void* block = nullptr;     // one device allocation holding every buffer
   size_t block_bytes = 0;
   float* staging = nullptr;  // W*H*4 bytes, reference-order copies
+  void* trace_buf = nullptr;  // path-line output, grown on demand
+  size_t trace_cap = 0;
   ProjTuning tuning;  // projection kernel shape, chosen at create
   osy_batch* batch = nullptr;  // the batch stepping this handle, if any
   struct Graph {
@@ -45,6 +47,7 @@ struct osy_fluid {
       for (auto& g : row)
         if (g.exec) (void)hipGraphExecDestroy(g.exec);
     if (block) (void)hipFree(block);
+    if (trace_buf) (void)hipFree(trace_buf);
     if (stream) (void)hipStreamDestroy(stream);
   }
 

@@ -58,4 +58,24 @@ class Json {
   friend class JsonParser;
 };
 
+// ConfigParser::get_or (inc/config_parser.hpp:131-150): a direct hit on the
+// whole key wins; otherwise split at the first '.' and look the rest up in
+// the child object.  A conversion error on a direct hit propagates (at the
+// top level the reference dies with an uncaught exception; the config loader
+// turns it into OSY_E_CONFIG); any error below the top level yields the
+// default.
+template <typename T, typename Conv>
+T json_get_or(const Json& j, const std::string& key, T def, Conv conv) {
+  if (j.contains(key)) return conv(j.at(key));
+  if (key.empty()) return def;
+  size_t dot = key.find('.');
+  if (dot == std::string::npos) return def;
+  std::string parent = key.substr(0, dot), child = key.substr(dot + 1);
+  try {
+    return json_get_or(j.at(parent), child, def, conv);
+  } catch (...) {
+    return def;
+  }
+}
+
 }  // namespace osy

@@ -141,6 +141,19 @@ void enqueue_export(int field, int storage, const StepParams& sp,
 void enqueue_import(int field, int storage, const StepParams& sp,
                     const FieldPtrs& f, int uv, int sm, const void* ref, hipStream_t s);
 
+// Frame export (update_fluid_pixels_kernel, src/graphics_handler.cu:258-285):
+// W*H RGBA8888 pixels in the reference's indx order into the device buffer
+// `out`, using the pressure range in f.prange.  pressure_on / smoke_on are the
+// config's enable_pressure / enable_smoke.
+void enqueue_render(int storage, const StepParams& sp, const FieldPtrs& f, int sm,
+                    int pressure_on, int smoke_on, uint32_t* out, hipStream_t s);
+
+// Path lines (update_traces_kernel, src/graphics_handler.cu:384-396) for the
+// seed grid of spacing `distance`: (W/distance)*(H/distance)*length int32
+// entries into each of the device buffers tx, ty.
+void enqueue_traces(int storage, const StepParams& sp, const FieldPtrs& f, int uv, int distance,
+                    int length, float d_t, int32_t* tx, int32_t* ty, hipStream_t s);
+
 // Plain device-to-device copy of `bytes` (a multiple of 16) with 16-byte
 // loads and stores, for calibrating hardware byte counters.
 void enqueue_copy(void* dst, const void* src, size_t bytes, hipStream_t s);

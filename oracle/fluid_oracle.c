@@ -302,11 +302,18 @@ void orc_render_rgba(const osy_config* c, const int32_t* solid, const float* smo
     out[k] = orc_pixel(solid[k], c->enable_pressure, c->enable_smoke, smoke[k], p[k], pmin, pmax);
 }
 
-/* ---- path lines: Fluid::trace (src/fluid.cu:16-36) ----------------------
- * From the centre of cell (i, j), `length` points: position += velocity *
- * d_t with (GX, GY) at each position; each point stored as
- * (round(x), H - 1 - round(y)), round half away from zero, converted like
- * the GPU's convert instruction (saturate, NaN to 0). */
+/* ---- path lines: Fluid::trace (src/fluid.cu:16-36) driven by
+ * update_traces_kernel / update_trace_at (src/graphics_handler.cu:367-396) --
+ * Seeds every `distance` cells, (ti * distance, tj * distance) for
+ * ti < W / distance, tj < H / distance; seed (ti, tj) owns the `length`
+ * entries from indx_traces(ti, tj, 0) = ((th - 1 - tj) * tw + ti) * length
+ * (src/graphics_handler.cu:7-11).  A seed on a solid cell gets -1 in every
+ * entry (the reference writes -1 to entry 0, which is what its drawing loop
+ * tests, and leaves the rest unwritten).  Otherwise the path starts at the
+ * cell centre and moves by (GX, GY) * d_t per point; each point is stored as
+ * (round(x), H - 1 - round(y)), round half away from zero, with the float ->
+ * int conversion of the GPU's convert instruction (saturate, NaN to 0; the
+ * reference's static_cast is undefined outside the int range). */
 static int32_t orc_sat_i32(float x) {
   if (x != x) return 0;
   if (x >= 2147483648.0f) return 2147483647;
@@ -314,25 +321,40 @@ static int32_t orc_sat_i32(float x) {
   return (int32_t)x;
 }
 
-void orc_trace(const osy_config* c, const int32_t* solid, const int32_t* ts, const float* u,
-               const float* v, const int32_t* seeds, int count, int length, float d_t,
-               int32_t* out) {
+static void orc_trace_one(const orc_grid* g, const float* u, const float* v, int i, int j,
+                          int length, float d_t, int32_t* tx, int32_t* ty) {
+  float x = orc_center_f32(i, g->cs);
+  float y = orc_center_f32(j, g->cs);
+  for (int k = 0; k < length; k++) {
+    if (k > 0) {
+      const float vx = orc_gx_f32(g, u, x, y);
+      const float vy = orc_gy_f32(g, v, x, y);
+      x = x + vx * d_t;
+      y = y + vy * d_t;
+    }
+    tx[k] = orc_sat_i32(roundf(x));
+    ty[k] = (int32_t)(uint32_t)((long)g->H - 1 - orc_sat_i32(roundf(y)));
+  }
+}
+
+int orc_trace_grid(const osy_config* c, const int32_t* solid, const int32_t* ts, const float* u,
+                   const float* v, int distance, int length, float d_t, int32_t* tx,
+                   int32_t* ty) {
+  if (distance < 1 || length < 1) return -1;
   ORC_GRID(c, solid, ts);
-  for (int s = 0; s < count; s++) {
-    float x = orc_center_f32(seeds[2 * s], g.cs);
-    float y = orc_center_f32(seeds[2 * s + 1], g.cs);
-    for (int k = 0; k < length; k++) {
-      if (k > 0) {
-        const float vx = orc_gx_f32(&g, u, x, y);
-        const float vy = orc_gy_f32(&g, v, x, y);
-        x = x + vx * d_t;
-        y = y + vy * d_t;
+  const int tw = g.W / distance, th = g.H / distance;
+  for (int tj = 0; tj < th; tj++) {
+    for (int ti = 0; ti < tw; ti++) {
+      const long o = ((long)(th - 1 - tj) * tw + ti) * length;
+      const int i = ti * distance, j = tj * distance;
+      if (solid[orc_idx(&g, i, j)]) {
+        for (int k = 0; k < length; k++) tx[o + k] = ty[o + k] = -1;
+      } else {
+        orc_trace_one(&g, u, v, i, j, length, d_t, tx + o, ty + o);
       }
-      int32_t* o = out + 2 * ((long)s * length + k);
-      o[0] = orc_sat_i32(roundf(x));
-      o[1] = g.H - 1 - orc_sat_i32(roundf(y));
     }
   }
+  return 0;
 }
 
 /* Lets a loader check that it opened this library and not another. */

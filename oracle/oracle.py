@@ -81,6 +81,14 @@ def _declare(L):
 
     L.orc_round16_array.restype = None
     L.orc_round16_array.argtypes = [_P, ctypes.c_long, ctypes.c_int]
+    L.orc_pixel.restype = ctypes.c_uint32
+    L.orc_pixel.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_float,
+                            ctypes.c_float, ctypes.c_float, ctypes.c_float]
+    L.orc_render_rgba.restype = None
+    L.orc_render_rgba.argtypes = [_P, _P, _P, _P, ctypes.c_float, ctypes.c_float, _P]
+    L.orc_trace_grid.restype = ctypes.c_int
+    L.orc_trace_grid.argtypes = [_P, _P, _P, _P, _P, ctypes.c_int, ctypes.c_int, ctypes.c_float,
+                                 _P, _P]
 
 
 def _ptr(a):
@@ -93,6 +101,47 @@ def round16(a: np.ndarray, storage: int) -> np.ndarray:
     out = np.ascontiguousarray(a, dtype=np.float32).copy()
     lib().orc_round16_array(_ptr(out), out.size, storage)
     return out
+
+
+def pixel(solid: bool, pressure_on: bool, smoke_on: bool, smoke: float, p: float,
+          pmin: float, pmax: float) -> int:
+    """One RGBA8888 pixel of update_fluid_pixels_kernel
+    (src/graphics_handler.cu:258-285)."""
+    return int(lib().orc_pixel(int(solid), int(pressure_on), int(smoke_on), smoke, p, pmin, pmax))
+
+
+def render_rgba(cfg, solid, smoke, p, pmin: float, pmax: float) -> np.ndarray:
+    """The frame of update_fluid_pixels_kernel as (H, W) uint32 RGBA8888
+    (r << 24 | g << 16 | b << 8 | a), row 0 the top of the screen, from
+    float32 smoke and pressure in the reference's memory order."""
+    H, W = cfg.height, cfg.width
+    solid = np.ascontiguousarray(solid, np.int32)
+    smoke = np.ascontiguousarray(smoke, np.float32)
+    p = np.ascontiguousarray(p, np.float32)
+    assert solid.shape == smoke.shape == p.shape == (H, W)
+    out = np.zeros((H, W), np.uint32)
+    lib().orc_render_rgba(ctypes.byref(cfg), _ptr(solid), _ptr(smoke), _ptr(p), pmin, pmax,
+                          _ptr(out))
+    return out
+
+
+def trace_grid(cfg, solid, total_s, u, v, distance: int, length: int, d_t: float):
+    """Path lines of update_traces_kernel (src/graphics_handler.cu:384-396):
+    (traces_x, traces_y), each (H // distance, W // distance, length) int32
+    in indx_traces order (row 0 holds the top row of seeds)."""
+    if distance < 1 or length < 1:
+        raise ValueError("distance and length must be at least 1")
+    th, tw = cfg.height // distance, cfg.width // distance
+    tx = np.zeros((th, tw, length), np.int32)
+    ty = np.zeros((th, tw, length), np.int32)
+    u = np.ascontiguousarray(u, np.float32)
+    v = np.ascontiguousarray(v, np.float32)
+    rc = lib().orc_trace_grid(ctypes.byref(cfg), _ptr(np.ascontiguousarray(solid, np.int32)),
+                              _ptr(np.ascontiguousarray(total_s, np.int32)), _ptr(u), _ptr(v),
+                              int(distance), int(length), float(d_t), _ptr(tx), _ptr(ty))
+    if rc != 0:
+        raise ValueError("distance and length must be at least 1")
+    return tx, ty
 
 
 def build_masks(cfg):
@@ -210,6 +259,17 @@ class OracleFluid:
     def idw(self, x, y):
         return self._fn("orc_point_idw")(self._c(), _ptr(self.solid), _ptr(self.total_s),
                                          _ptr(self.smoke), self.ctype(x), self.ctype(y))
+
+    def render_rgba(self):
+        """The frame for the current state, with the pressure range of the
+        last update (as the reference passes fluid.min_pressure / max_pressure)."""
+        pmin, pmax = self.pressure_range
+        return render_rgba(self.cfg, self.solid, self.smoke, self.p, pmin, pmax)
+
+    def traces(self, distance: int, length: int, d_t=None):
+        dt = self.cfg.d_t if d_t is None else d_t
+        return trace_grid(self.cfg, self.solid, self.total_s, self.u, self.v, distance, length,
+                          dt)
 
     def divergence_norm(self):
         out = np.zeros(2, np.float64)

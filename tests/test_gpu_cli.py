@@ -1,0 +1,90 @@
+"""The headless fluid-simulator program on the GPU: its .npy field dumps,
+frames and path lines equal what the Python API computes for the same
+config and steps (both call Fluid::update once per frame through the same C
+ABI), and its performance record is valid JSON."""
+import json
+import subprocess
+
+import numpy as np
+import pytest
+
+from opensayal_amd import ConfigParser, Fluid
+from opensayal_amd._abi import CLI_PATH
+
+pytestmark = pytest.mark.gpu
+
+CONFIG = {
+    "sim": {
+        "width": 130, "height": 74, "cell_size": 1, "enable_pressure": True,
+        "enable_smoke": True, "enable_drain": True,
+        "projection": {"n": 15, "o": 1.9},
+        "wind_tunnel": {"speed": 2.0, "pipe_height": 30, "smoke_height": 10,
+                        "smoke_count": 2, "smoke_length": 2, "smoke": 1.0},
+        "physics": {"g": -5.0},
+        "time": {"d_t": 0.04},
+        "smoke": {"enable_decay": True, "decay_rate": 0.2},
+        "obstacle": {"enable": True, "center_x": 50, "center_y": 37, "radius": 9.5},
+    },
+    "fluid": {"density": 1.0, "drag_coeff": 0.1, "viscosity": 0},
+    "visual": {"path_line": {"enable": True, "length": 12, "distance": 9}},
+}
+
+
+def run_cli(*args, cwd=None):
+    r = subprocess.run([str(CLI_PATH), *args], capture_output=True, text=True, timeout=120,
+                       cwd=cwd)
+    assert r.returncode == 0, r.stderr
+    return r
+
+
+def load(d, step, name):
+    return np.load(d / f"step_{step:06d}_{name}.npy", allow_pickle=False)
+
+
+@pytest.mark.parametrize("storage", ["f32", "f16"])
+def test_cli_dumps_equal_python_api(tmp_path, storage):
+    cfg_json = json.loads(json.dumps(CONFIG))
+    cfg_json["sim"]["storage"] = storage
+    path = tmp_path / "OpenSayal.conf.json"
+    path.write_text(json.dumps(cfg_json))
+    out = tmp_path / "dump"
+    r = run_cli("--config", str(path), "--steps", "7", "--dump", str(out), "--dump-every", "3",
+                "--frame", "--save-report", "--report", str(tmp_path / "r.json"))
+    record = json.loads(r.stdout.strip().splitlines()[-1])
+    assert record["steps"] == 7 and record["width"] == 130 and record["storage"] == storage
+    assert record["gcell_steps_per_s"] > 0
+    assert json.loads((tmp_path / "r.json").read_text()) == record
+
+    cfg = ConfigParser(str(path)).parse()
+    fl = Fluid(cfg)
+    for step in range(1, 8):
+        fl.update()
+        if step not in (3, 6, 7):
+            continue
+        for name in ("vel_x", "vel_y", "smoke", "pressure"):
+            got = load(out, step, name)
+            assert got.dtype == np.float32 and got.shape == (74, 130)
+            assert np.array_equal(got.view(np.uint32), fl.get_field(name).view(np.uint32)), \
+                (step, name)
+        frame = load(out, step, "frame")
+        assert frame.dtype == np.uint32
+        assert np.array_equal(frame, fl.render_rgba()), step
+        tx, ty = fl.traces(9, 12, d_t=cfg.d_t)
+        assert np.array_equal(load(out, step, "traces_x"), tx)
+        assert np.array_equal(load(out, step, "traces_y"), ty)
+        ppm = (out / f"step_{step:06d}_frame.ppm").read_bytes()
+        head = b"P6\n130 74\n255\n"
+        assert ppm.startswith(head)
+        rgb = np.frombuffer(ppm[len(head):], np.uint8).reshape(74, 130, 3)
+        assert np.array_equal(rgb[..., 0], (frame >> 24).astype(np.uint8))
+        assert np.array_equal(rgb[..., 2], ((frame >> 8) & 255).astype(np.uint8))
+
+
+def test_cli_compare_performance(tmp_path):
+    path = tmp_path / "OpenSayal.conf.json"
+    path.write_text(json.dumps(CONFIG))
+    run_cli("--steps", "12", "--save-report", cwd=tmp_path)  # default names in cwd
+    assert "avg_frame_us" in json.loads((tmp_path / "report.json").read_text())
+    r = run_cli("--steps", "12", "--compare-performance", "--log-every", "4", cwd=tmp_path)
+    assert "performance gain:" in r.stdout
+    assert r.stdout.count("FPS:") == 3
