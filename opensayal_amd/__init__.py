@@ -24,7 +24,7 @@ import numpy as np
 from ._abi import (FIELDS, STORAGE, OpenSayalError, OsyBatchInfo, OsyBatchTuning, OsyConfig,
                    OsyFluidInfo, OsySource, OsyTuning, check, lib)
 
-__all__ = ["Batch", "Config", "ConfigParser", "Event", "Fluid", "Source", "Stream",
+__all__ = ["Batch", "Config", "ConfigParser", "Event", "Fluid", "Mouse", "Source", "Stream",
            "OpenSayalError", "default_config", "device_bytes", "FIELDS", "STORAGE"]
 
 Config = OsyConfig
@@ -306,6 +306,55 @@ class Fluid:
                                      tx.ctypes.data_as(ctypes.c_void_p),
                                      ty.ctypes.data_as(ctypes.c_void_p), tx.nbytes))
         return tx, ty
+
+
+INTERACTIVE_STARTING_SPEED = 200  # inc/config.hpp:10
+
+
+def _trunc_div(a: int, b: int) -> int:
+    """C integer division (truncates toward zero)."""
+    q = abs(a) // abs(b)
+    return q if (a >= 0) == (b >= 0) else -q
+
+
+class Mouse:
+    """Mouse (src/mouse.cu:4-45) without SDL: a window's event loop feeds it
+    button, wheel and motion events, and each frame asks for the Source to
+    pass to Fluid.update (src/main.cu:85-88)."""
+
+    def __init__(self):
+        self.is_down = False
+        self.button = 0
+        self.wheel_value = INTERACTIVE_STARTING_SPEED
+        self.x = 0
+        self.y = 0
+
+    def button_down(self, button: int) -> None:  # SDL_MOUSEBUTTONDOWN
+        self.is_down = True
+        self.button = int(button)
+
+    def button_up(self) -> None:  # SDL_MOUSEBUTTONUP
+        self.is_down = False
+
+    def wheel(self, y: int) -> None:  # SDL_MOUSEWHEEL: 10 per notch, never below 0
+        value = self.wheel_value + int(y) * 10
+        if value >= 0:
+            self.wheel_value = value
+
+    def motion(self, x: int, y: int) -> None:  # SDL_MOUSEMOTION, window pixels
+        self.x, self.y = int(x), int(y)
+
+    def make_source(self, fluid_height: int, cell_size: int) -> Source:
+        """Mouse::make_source: the cell under the pointer (window y grows
+        downwards, fluid j upwards), smoke with the left button, the wheel's
+        speed, pointing backwards with the middle button."""
+        src = Source()
+        src.active = int(self.is_down)
+        src.smoke = 1.0 if self.button == 1 else 0.0
+        src.velocity = float(-self.wheel_value if self.button == 2 else self.wheel_value)
+        src.position_x = _trunc_div(self.x, cell_size)
+        src.position_y = fluid_height - _trunc_div(self.y, cell_size)
+        return src
 
 
 def device_bytes(config: Config) -> int:
