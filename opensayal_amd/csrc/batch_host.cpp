@@ -328,6 +328,9 @@ int osy_batch_destroy(osy_batch* b) {
 int osy_batch_add(osy_batch* b, osy_fluid* fl) {
   if (!b || !fl) return set_error(OSY_E_INVALID, "null argument");
   if (fl->batch) return set_error(OSY_E_STATE, "the handle is already in a batch");
+  if (fl->cfg.viscosity != 0.0f)
+    return set_error(OSY_E_UNSUPPORTED,
+                     "a batch does not run the diffusion pass: members need fluid.viscosity 0");
   if (b->budget && b->used + fl->block_bytes > b->budget)
     return set_error(OSY_E_BUDGET, "the handle's " + std::to_string(fl->block_bytes) +
                                        " device bytes exceed the batch budget (" +

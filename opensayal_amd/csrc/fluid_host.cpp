@@ -39,11 +39,8 @@ int config_validate(const osy_config& c) {
     return set_error(OSY_E_INVALID, "sim.projection.n must be in [0, 100000]");
   if (!(c.projection_o >= 0.0f && c.projection_o <= 2.0f))
     return set_error(OSY_E_INVALID, "sim.projection.o must be within [0, 2]");
-  if (c.viscosity != 0.0f)
-    return set_error(OSY_E_UNSUPPORTED,
-                     "fluid.viscosity must be 0: the reference's diffusion pass "
-                     "(src/fluid.cu:167-190) is racy and reads an uninitialised "
-                     "loop counter, so it has no defined result to reproduce");
+  if (!std::isfinite(c.viscosity))
+    return set_error(OSY_E_INVALID, "fluid.viscosity must be finite");
   if (c.smoke_count != 1) {
     const int spacing = (c.smoke_count - 1) != 0
                             ? (c.pipe_height - c.smoke_count * c.smoke_height) / (c.smoke_count - 1)

@@ -92,6 +92,9 @@ struct osy_fluid {
     sp.smoke_on = c.enable_smoke && c.wind_smoke != 0.0f;
     sp.decay_on = c.enable_decay != 0;
     sp.decay_rd = c.decay_rate * d_t;
+    sp.visc_on = c.viscosity != 0.0f;
+    sp.visc_a = c.viscosity * d_t / (float)(cs * cs);
+    sp.visc_den = 1.0f + 4.0f * sp.visc_a;
     return sp;
   }
 };

@@ -37,6 +37,10 @@ struct StepParams {
   int smoke_on;           // enable_smoke && wind_tunnel.smoke != 0
   int decay_on;
   float decay_rd;         // decay_rate * d_t
+  // apply_diffusion_at (src/fluid.cu:176-183): a = viscosity * d_t / cs^2
+  // and the divisor 1 + 4a, both in float as the reference computes them
+  int visc_on;
+  float visc_a, visc_den;
   // set by the step: the first projection launch applies the velocity
   // forces to what it loads and starts the pressure sums at 0
   int forces_in_projection;

@@ -183,6 +183,84 @@ __global__ void __launch_bounds__(64 * kAdvNY)
                                             tile / ntx, sh);
 }
 
+// ---- P2b: apply_diffusion (src/fluid.cu:167-190) as Jacobi rounds -------
+// Every interior cell (1 <= i <= W-2, 1 <= j <= H-2, solid or not) becomes
+// (u + a*(((u(i-1,j) + u(i+1,j)) + u(i,j-1)) + u(i,j+1))) / (1 + 4a) of the
+// previous round's u; borders keep theirs; v is not diffused.  A launch runs
+// up to kDiffHalo rounds on a 64 x 64 tile staged with a kDiffHalo-cell halo
+// in LDS (two buffers, ping-pong): the values at region distance d from the
+// region edge are exact for d rounds, so the owned tile is exact after
+// kDiffHalo rounds.  u goes from u_in to u_out; with copy_v the launch also
+// moves v to v_out so that u and v stay in the same buffer of their pairs.
+constexpr int kDiffT = 64, kDiffHalo = 6, kDiffR = kDiffT + 2 * kDiffHalo;
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_diffuse(StepParams sp, int rounds, const T* __restrict__ u_in, T* __restrict__ u_out,
+              const T* __restrict__ v_in, T* __restrict__ v_out, int copy_v, int ntx) {
+  __shared__ float buf[2][kDiffR * kDiffR];
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int x0 = (tile % ntx) * kDiffT - kDiffHalo, y0 = (tile / ntx) * kDiffT - kDiffHalo;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int r = wave; r < kDiffR; r += 4) {
+    const int j = y0 + r;
+    for (int c = lane; c < kDiffR; c += 64) {
+      const int i = x0 + c;
+      const bool in = (unsigned)i < (unsigned)sp.W && (unsigned)j < (unsigned)sp.H;
+      const float x = in ? ldf(u_in, (long)j * sp.P + i) : 0.0f;
+      buf[0][r * kDiffR + c] = x;
+      buf[1][r * kDiffR + c] = x;
+    }
+  }
+  __syncthreads();
+  const float a = sp.visc_a, den = sp.visc_den;
+  for (int it = 0; it < rounds; it++) {
+    const float* src = buf[it & 1];
+    float* dst = buf[(it & 1) ^ 1];
+    for (int r = 1 + wave; r < kDiffR - 1; r += 4) {
+      const int j = y0 + r;
+      if (j < 1 || j > sp.H - 2) continue;
+      for (int c = 1 + lane; c < kDiffR - 1; c += 64) {
+        const int i = x0 + c;
+        if (i < 1 || i > sp.W - 2) continue;
+        const int l = r * kDiffR + c;
+        const float sum = ((src[l - 1] + src[l + 1]) + src[l - kDiffR]) + src[l + kDiffR];
+        dst[l] = rnd<T>((src[l] + a * sum) / den);
+      }
+    }
+    __syncthreads();
+  }
+  const float* res = buf[rounds & 1];
+  for (int r = kDiffHalo + wave; r < kDiffHalo + kDiffT; r += 4) {
+    const int j = y0 + r;
+    if (j >= sp.H) break;
+    const int i = x0 + kDiffHalo + lane;
+    if (i >= sp.W) continue;
+    const long k = (long)j * sp.P + i;
+    stf(u_out, k, res[r * kDiffR + kDiffHalo + lane]);
+    if (copy_v) v_out[k] = v_in[k];
+  }
+}
+
+// `sp.n` rounds in ceil(n / kDiffHalo) launches; advances *uv by the launch
+// count (v moves with u on the last launch when that count is odd).
+template <typename T>
+void enqueue_diffusion(const StepParams& sp, const FieldPtrs& f, int* uv, hipStream_t s) {
+  if (sp.n <= 0) return;
+  const int ntx = (sp.W + kDiffT - 1) / kDiffT, nty = (sp.H + kDiffT - 1) / kDiffT;
+  const int launches = (sp.n + kDiffHalo - 1) / kDiffHalo;
+  int cur = *uv;
+  for (int q = 0; q < launches; q++) {
+    const int rounds = q + 1 < launches ? kDiffHalo : sp.n - kDiffHalo * (launches - 1);
+    const int copy_v = (q + 1 == launches) && (launches & 1);
+    hipLaunchKernelGGL(k_diffuse<T>, dim3(ntx * nty), dim3(256), 0, s, sp, rounds,
+                       (const T*)f.u[cur], (T*)f.u[1 - cur], (const T*)f.v[cur],
+                       (T*)f.v[1 - cur], copy_v, ntx);
+    cur = 1 - cur;
+  }
+  *uv = cur;
+}
+
 // ---- divergence norms (Fluid::get_divergence, src/fluid.cu:192-201) -------
 __device__ inline double wave_sum_d(double x) {
   for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
@@ -286,7 +364,11 @@ void step_t(const StepParams& sp_in, const FieldPtrs& f, int* uv, int* sm,
   // the first tiled projection launch applies the velocity forces (and
   // starts the pressure sums at 0) as it loads; the smoke advection applies
   // the smoke forces as it loads
-  const bool fuse_uv = sp.n > 0 && tuning.kernel != PROJ_COLOUR && tuning.valid(sp.pressure_on);
+  // (with diffusion the forces come first as a pass of their own, then the
+  // diffusion launches, src/fluid.cu:771-776)
+  const bool diffuse = sp.visc_on && sp.n > 0;
+  const bool fuse_uv =
+      !diffuse && sp.n > 0 && tuning.kernel != PROJ_COLOUR && tuning.valid(sp.pressure_on);
   const bool fuse_smoke = sp.smoke_on;
   sp.forces_in_projection = fuse_uv ? 1 : 0;
   if (!fuse_uv || !fuse_smoke)
@@ -294,6 +376,7 @@ void step_t(const StepParams& sp_in, const FieldPtrs& f, int* uv, int* sm,
                        fuse_uv ? 0 : 1, fuse_smoke ? 0 : 1, (T*)f.u[*uv], (T*)f.v[*uv],
                        (T*)f.smoke[*sm], f.p);
   int cur = *uv;
+  if (diffuse) enqueue_diffusion<T>(sp, f, &cur, s);
   enqueue_projection<T>(sp, g, f, &cur, tuning, s);
   if (sp.pressure_on) {
     hipLaunchKernelGGL(k_minmax_partial, dim3(kReduceBlocks), dim3(256), 0, s, sp, f.p, f.scratch);

@@ -201,6 +201,12 @@ int orc_build_masks(const osy_config* c, int32_t* is_solid, int32_t* total_s) {
     ORC_GRID(c, solid, ts);                                                    \
     orc_decay##S(&g, smoke, rate, d_t, 1);                                     \
   }                                                                            \
+  void orc_phase_diffusion##S(const osy_config* c, const int32_t* solid,       \
+                              const int32_t* ts, R* u, R* tmp, R d_t,          \
+                              int sweeps) {                                    \
+    ORC_GRID(c, solid, ts);                                                    \
+    orc_diffusion##S(c, &g, u, tmp, d_t, sweeps, 1);                           \
+  }                                                                            \
   R orc_point_gx##S(const osy_config* c, const int32_t* solid,                 \
                     const int32_t* ts, const R* u, R x, R y) {                 \
     ORC_GRID(c, solid, ts);                                                    \

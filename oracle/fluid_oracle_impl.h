@@ -374,6 +374,37 @@ void FN(orc_decay)(const orc_grid* g, REAL* smoke, REAL rate, REAL d_t,
   for (long k = 0; k < n; k++) smoke[k] = (REAL)fmax((double)(smoke[k] - rd), 0.0);
 }
 
+/* apply_diffusion (src/fluid.cu:167-190), restated as Jacobi: `sweeps`
+ * rounds in which every interior cell (1 <= i <= W-2, 1 <= j <= H-2; solid
+ * or not, as the reference's kernel does) becomes
+ *   (u + a * (((u(i-1,j) + u(i+1,j)) + u(i,j-1)) + u(i,j+1))) / (1 + 4a)
+ * from the previous round's values; borders keep theirs; v is untouched.
+ * a = viscosity * d_t / (cs * cs).  The reference updates u in place from
+ * concurrent threads (reads may see either value) and counts its loop with
+ * an uninitialised variable; the round count here is the loop's evident
+ * bound, projection n, and every read sees the previous round.  `tmp`
+ * holds W*H REAL values. */
+void FN(orc_diffusion)(const osy_config* c, const orc_grid* g, REAL* u, REAL* tmp, REAL d_t,
+                       int sweeps, int nthreads) {
+  const int W = g->W, H = g->H;
+  const REAL a = (REAL)c->viscosity * d_t / (REAL)(g->cs * g->cs);
+  const REAL den = (REAL)1 + (REAL)4 * a;
+  const long n = (long)W * H;
+  for (int it = 0; it < sweeps; it++) {
+    memcpy(tmp, u, sizeof(REAL) * n);
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+    for (int j = 1; j <= H - 2; j++) {
+      for (int i = 1; i <= W - 2; i++) {
+        const REAL sum = ((tmp[orc_idx(g, i - 1, j)] + tmp[orc_idx(g, i + 1, j)]) +
+                          tmp[orc_idx(g, i, j - 1)]) +
+                         tmp[orc_idx(g, i, j + 1)];
+        u[orc_idx(g, i, j)] = (tmp[orc_idx(g, i, j)] + a * sum) / den;
+      }
+    }
+    FN(orc_round)(u, n, c->storage);
+  }
+}
+
 /* Fluid::update (src/fluid.cu:770-795).  `scratch` holds 3*W*H REAL values
  * for the advection outputs.  Writes the pressure range when pressure is
  * enabled (thrust::reduce min/max, src/fluid.cu:778-787). */
@@ -389,6 +420,7 @@ void FN(orc_update)(const osy_config* c, const orc_grid* g, REAL* u, REAL* v,
   FN(orc_round)(u, n, c->storage);
   FN(orc_round)(v, n, c->storage);
   FN(orc_round)(smoke, n, c->storage);
+  if (c->viscosity != 0) FN(orc_diffusion)(c, g, u, scratch, d_t, c->projection_n, nthreads);
   FN(orc_projection)(c, g, u, v, p, d_t, c->projection_n, nthreads);
   if (c->enable_pressure && prange) {
     REAL lo = (REAL)INFINITY, hi = (REAL)-INFINITY;

@@ -79,6 +79,10 @@ def _declare(L):
         f.argtypes = [_P, _P, _P, _P, _P, _P]
 
 
+    for sfx, real in (("_f32", ctypes.c_float), ("_f64", ctypes.c_double)):
+        f = getattr(L, "orc_phase_diffusion" + sfx)
+        f.restype = None
+        f.argtypes = [_P, _P, _P, _P, _P, real, ctypes.c_int]
     L.orc_round16_array.restype = None
     L.orc_round16_array.argtypes = [_P, ctypes.c_long, ctypes.c_int]
     L.orc_pixel.restype = ctypes.c_uint32
@@ -247,6 +251,14 @@ class OracleFluid:
         r = self.cfg.decay_rate if rate is None else rate
         self._fn("orc_phase_decay")(self._c(), _ptr(self.solid), _ptr(self.total_s),
                                     _ptr(self.smoke), self.ctype(r), self.ctype(dt))
+
+    def diffusion(self, sweeps=None, d_t=None):
+        """The diffusion pass alone (Jacobi rounds on u)."""
+        dt = self.cfg.d_t if d_t is None else d_t
+        n = self.cfg.projection_n if sweeps is None else sweeps
+        self._fn("orc_phase_diffusion")(self._c(), _ptr(self.solid), _ptr(self.total_s),
+                                        _ptr(self.u), _ptr(self._scratch), self.ctype(dt),
+                                        int(n))
 
     def gx(self, x, y):
         return self._fn("orc_point_gx")(self._c(), _ptr(self.solid), _ptr(self.total_s),

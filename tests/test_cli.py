@@ -51,8 +51,9 @@ def test_missing_default_config(tmp_path):
     (json.dumps({"sim": {"storage": "f64"}}), "sim.storage"),
     # a direct hit on the whole dotted key converts or fails (get_or)
     (json.dumps({"sim.width": "wide"}), "config"),
-    # the reference's default viscosity would start its racy diffusion pass
-    (json.dumps({"sim": {"width": 64, "height": 48}}), "fluid.viscosity must be 0"),
+    # a non-finite viscosity is refused when the solver is created
+    (json.dumps({"sim": {"width": 64, "height": 48}, "fluid": {"viscosity": 1e39}}),
+     "fluid.viscosity must be finite"),
 ])
 def test_config_errors(tmp_path, text, msg):
     cfg = tmp_path / "c.json"

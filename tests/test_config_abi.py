@@ -118,7 +118,7 @@ def test_storage_key():
 
 
 @pytest.mark.parametrize("bad,status", [
-    (dict(viscosity=0.001), _abi.OSY_E_UNSUPPORTED),
+    (dict(viscosity=float("nan")), _abi.OSY_E_INVALID),
     (dict(block_size_x=64, obstacle_radius=2.0e7), _abi.OSY_E_INVALID),
     (dict(projection_o=2.5), _abi.OSY_E_INVALID),
     (dict(projection_n=-1), _abi.OSY_E_INVALID),

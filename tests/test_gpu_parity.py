@@ -269,7 +269,7 @@ def test_invalid_tuning_rejected():
 
 def test_errors_are_reported():
     with pytest.raises(OpenSayalError) as e:
-        Fluid(make_config(64, 36, viscosity=0.001))
+        Fluid(make_config(64, 36, viscosity=float("inf")))
     assert "viscosity" in str(e.value)
     fl = Fluid(make_config(64, 36))
     with pytest.raises(OpenSayalError):
