@@ -53,8 +53,23 @@ def host_threads() -> int:
     return max(1, min(n, 16))
 
 
-def cpu_baseline(cfg, seed: int, budget_s: float = 12.0) -> dict:
-    """Times the oracle (FP32, OpenMP) on this host for a bounded sample."""
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(cfg, seed: int, budget_s: float = 15.0) -> dict:
+    """Times the oracle (FP32, OpenMP over rows, threads bound to cores) on
+    this host: three repetitions of a bounded number of steps, median."""
+    # the OpenMP runtime reads these when the oracle library first runs
+    os.environ.setdefault("OMP_PLACES", "cores")
+    os.environ.setdefault("OMP_PROC_BIND", "close")
     from opensayal_amd import workloads as wl
     from oracle.oracle import OracleFluid
 
@@ -65,20 +80,25 @@ def cpu_baseline(cfg, seed: int, budget_s: float = 12.0) -> dict:
     t0 = time.perf_counter()
     o.update(steps=1)
     first = time.perf_counter() - t0
-    steps = max(2, min(40, int(budget_s / max(first, 1e-6))))
-    t0 = time.perf_counter()
-    o.update(steps=steps)
-    dt = time.perf_counter() - t0
+    steps = max(2, min(40, int(budget_s / 3 / max(first, 1e-6))))
+    reps = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        o.update(steps=steps)
+        reps.append(time.perf_counter() - t0)
+    dt = sorted(reps)[1]
     cells = cfg.width * cfg.height
     return {
         "value": cells * steps / dt / 1e9,
         "unit": "Gcell-steps/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"{steps} steps of the same {cfg.width}x{cfg.height} n={cfg.projection_n} "
-                  f"workload after 1 untimed step ({dt:.1f} s), oracle FP32, "
-                  f"OpenMP {threads} threads",
+        "sample": f"median of 3 runs of {steps} steps of the same {cfg.width}x{cfg.height} "
+                  f"n={cfg.projection_n} workload after 1 untimed step "
+                  f"({sum(reps):.1f} s in all), oracle FP32, OpenMP {threads} threads",
         "ms_per_step": dt / steps * 1e3,
+        "cpu_model": cpu_model(),
+        "host_cpus": os.cpu_count(),
     }
 
 
@@ -150,6 +170,33 @@ def batch_leg(specs, steps: int, warmup: int, plain_steps: int = 3, autotune: bo
     }
 
 
+def diffusion_leg(seed: int, steps: int, warmup: int) -> dict:
+    """The same 1920x1080 step with the reference's default fluid.viscosity
+    (0.001): the forces pass, n = 50 Jacobi diffusion rounds (9 launches) and
+    then the plain step's projection and advections."""
+    from opensayal_amd import Event, Fluid, Stream, workloads as wl
+
+    cfg = wl.plain_full(viscosity=0.001)
+    fl = Fluid(cfg)
+    u, v, s = wl.initial_state(cfg, seed)
+    fl.set_field("vel_x", u)
+    fl.set_field("vel_y", v)
+    fl.set_field("smoke", s)
+    stream = Stream()
+    fl.autotune()
+    fl.step_async(warmup, stream=stream)
+    a, b = Event(), Event()
+    a.record(stream)
+    fl.step_async(steps, stream=stream)
+    b.record(stream)
+    stream.synchronize()
+    ms = a.elapsed_ms(b) / steps
+    fl.close()
+    return {"viscosity": 0.001, "steps": steps, "ms_per_step": round(ms, 4),
+            "value": round(cfg.width * cfg.height / (ms / 1e3) / 1e9, 4),
+            "unit": "Gcell-steps/s"}
+
+
 def view_leg(fl, reps: int) -> dict:
     """What GraphicsHandler reads each frame, as the synchronous library
     calls a viewer makes: the RGBA frame (kernel plus the W*H*4-byte copy to
@@ -217,6 +264,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--kernel-reps", type=int, default=20)
     ap.add_argument("--no-view", action="store_true", help="skip the frame export leg")
+    ap.add_argument("--no-diffusion", action="store_true",
+                    help="skip the leg with the default viscosity")
     ap.add_argument("--view-reps", type=int, default=20)
     ap.add_argument("--requests", type=int, default=64, help="serving: number of requests")
     ap.add_argument("--rate", type=float, default=8.0, help="serving: arrivals per second")
@@ -292,6 +341,14 @@ def main() -> int:
     wall = time.perf_counter() - t0
     gpu_ms = start.elapsed_ms(end)
     ms_per_step = gpu_ms / args.steps
+    # four more repetitions of the same K steps, for the spread (not `value`)
+    reps = [ms_per_step]
+    for _ in range(4):
+        start.record(stream)
+        fl.step_async(args.steps, stream=stream)
+        end.record(stream)
+        stream.synchronize()
+        reps.append(start.elapsed_ms(end) / args.steps)
     value = W * H * args.steps / (gpu_ms / 1e3) / 1e9
 
     # dominant kernel: kernel_reps projection launches of sweeps_per_launch
@@ -344,10 +401,14 @@ def main() -> int:
                           "achieved_GBs": round(step_alg, 1),
                           "frac": round(step_alg / HBM_PEAK_GBS, 4)},
         "wall_s": round(wall, 3),
+        "reps_ms_per_step": [round(x, 4) for x in reps],
+        "median_ms_per_step": round(sorted(reps)[2], 4),
     }
     if not args.no_view:
         out["frame_export"] = view_leg(fl, args.view_reps)
     fl.close()
+    if not args.no_diffusion and args.workload == "full":
+        out["with_diffusion"] = diffusion_leg(args.seed, args.steps // 4, args.warmup)
     if not args.no_batch and args.workload == "full":
         # 256 uneven grids (128..1024 wide, n from {20, 50, 100}) and 256
         # small ones (64..384 wide), each as one batch against the per-grid path

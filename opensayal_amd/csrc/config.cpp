@@ -74,9 +74,7 @@ void config_defaults(osy_config* c, int32_t width, int32_t height) {
 
 // ConfigParser::parse (src/config_parser.cpp:15-184).  Render-only keys
 // (visual.*) are accepted and not stored.  fluid.viscosity keeps the
-// reference's default of 0.001 when absent, so a config that does not set it
-// to 0 is rejected at create time, exactly where the reference would start
-// running its diffusion pass.
+// reference's default of 0.001 when absent, which runs the diffusion pass.
 int config_parse_text(const std::string& text, osy_config* out) {
   Json j;
   try {

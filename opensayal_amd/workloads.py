@@ -65,15 +65,17 @@ def initial_state(cfg: Config, seed: int):
 
 def wind_tunnel(width: int, height: int, *, pipe_height: int, cx: int, cy: int, radius: float,
                 speed: float = 2.0, n: int = 50, o: float = 1.9, g: float = -5.0,
-                d_t: float = 0.05, pressure: bool = False, storage: str = "f32") -> Config:
+                d_t: float = 0.05, pressure: bool = False, storage: str = "f32",
+                viscosity: float = 0.0) -> Config:
     """A wind-tunnel configuration with the benchmark's fixed choices: drain
     open (solid walls top, bottom and left), smoke 1 in a single band of the
-    default height, no decay, no drag, viscosity 0, cell size 1."""
+    default height, no decay, no drag, cell size 1; viscosity 0 (no diffusion
+    pass) unless given."""
     return default_config(
         width, height, pipe_height=pipe_height, obstacle_center_x=cx, obstacle_center_y=cy,
         obstacle_radius=radius, wind_speed=speed, projection_n=n, projection_o=o, g=g, d_t=d_t,
         enable_pressure=int(pressure), enable_drain=1, enable_smoke=1, wind_smoke=1.0,
-        smoke_count=1, smoke_length=1, enable_decay=0, drag_coeff=0.0, viscosity=0.0,
+        smoke_count=1, smoke_length=1, enable_decay=0, drag_coeff=0.0, viscosity=viscosity,
         density=1.0, storage=storage)
 
 
