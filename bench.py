@@ -80,7 +80,7 @@ def cpu_baseline(cfg, seed: int, budget_s: float = 15.0) -> dict:
     t0 = time.perf_counter()
     o.update(steps=1)
     first = time.perf_counter() - t0
-    steps = max(2, min(40, int(budget_s / 3 / max(first, 1e-6))))
+    steps = max(2, min(400, int(budget_s / 3 / max(first, 1e-6))))
     reps = []
     for _ in range(3):
         t0 = time.perf_counter()

@@ -187,58 +187,69 @@ __global__ void __launch_bounds__(64 * kAdvNY)
 // Every interior cell (1 <= i <= W-2, 1 <= j <= H-2, solid or not) becomes
 // (u + a*(((u(i-1,j) + u(i+1,j)) + u(i,j-1)) + u(i,j+1))) / (1 + 4a) of the
 // previous round's u; borders keep theirs; v is not diffused.  A launch runs
-// up to kDiffHalo rounds on a 64 x 64 tile staged with a kDiffHalo-cell halo
-// in LDS (two buffers, ping-pong): the values at region distance d from the
-// region edge are exact for d rounds, so the owned tile is exact after
-// kDiffHalo rounds.  u goes from u_in to u_out; with copy_v the launch also
+// up to kDiffHalo rounds on a 128 x 64 region (the owned tile plus a
+// kDiffHalo-cell halo) staged in LDS, two buffers in turn: a value at region
+// distance d from the region edge is exact for d rounds, so the owned tile
+// is exact after kDiffHalo rounds.  Eight wavefronts take the region's rows
+// in turn, each lane two columns 64 apart, so the 126 updated columns fill
+// 126 of 128 lanes.  u goes from u_in to u_out; with copy_v the launch also
 // moves v to v_out so that u and v stay in the same buffer of their pairs.
-constexpr int kDiffT = 64, kDiffHalo = 6, kDiffR = kDiffT + 2 * kDiffHalo;
+constexpr int kDiffRX = 128, kDiffRY = 64, kDiffHalo = 6;
+constexpr int kDiffTX = kDiffRX - 2 * kDiffHalo, kDiffTY = kDiffRY - 2 * kDiffHalo;
 
 template <typename T>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(512)
     k_diffuse(StepParams sp, int rounds, const T* __restrict__ u_in, T* __restrict__ u_out,
               const T* __restrict__ v_in, T* __restrict__ v_out, int copy_v, int ntx) {
-  __shared__ float buf[2][kDiffR * kDiffR];
+  __shared__ float buf[2][kDiffRY * kDiffRX];
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
-  const int x0 = (tile % ntx) * kDiffT - kDiffHalo, y0 = (tile / ntx) * kDiffT - kDiffHalo;
+  const int x0 = (tile % ntx) * kDiffTX - kDiffHalo, y0 = (tile / ntx) * kDiffTY - kDiffHalo;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (int r = wave; r < kDiffR; r += 4) {
+  for (int r = wave; r < kDiffRY; r += 8) {
     const int j = y0 + r;
-    for (int c = lane; c < kDiffR; c += 64) {
-      const int i = x0 + c;
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      const int c = lane + 64 * h, i = x0 + c;
       const bool in = (unsigned)i < (unsigned)sp.W && (unsigned)j < (unsigned)sp.H;
       const float x = in ? ldf(u_in, (long)j * sp.P + i) : 0.0f;
-      buf[0][r * kDiffR + c] = x;
-      buf[1][r * kDiffR + c] = x;
+      buf[0][r * kDiffRX + c] = x;
+      buf[1][r * kDiffRX + c] = x;
     }
   }
   __syncthreads();
   const float a = sp.visc_a, den = sp.visc_den;
+  // columns 1 + lane and 65 + lane (the second only for lanes 0..61)
+  const int c0 = 1 + lane, c1 = 65 + lane;
+  const bool in0 = x0 + c0 >= 1 && x0 + c0 <= sp.W - 2;
+  const bool in1 = c1 < kDiffRX - 1 && x0 + c1 >= 1 && x0 + c1 <= sp.W - 2;
   for (int it = 0; it < rounds; it++) {
     const float* src = buf[it & 1];
     float* dst = buf[(it & 1) ^ 1];
-    for (int r = 1 + wave; r < kDiffR - 1; r += 4) {
+    for (int r = 1 + wave; r < kDiffRY - 1; r += 8) {
       const int j = y0 + r;
       if (j < 1 || j > sp.H - 2) continue;
-      for (int c = 1 + lane; c < kDiffR - 1; c += 64) {
-        const int i = x0 + c;
-        if (i < 1 || i > sp.W - 2) continue;
-        const int l = r * kDiffR + c;
-        const float sum = ((src[l - 1] + src[l + 1]) + src[l - kDiffR]) + src[l + kDiffR];
+#pragma unroll
+      for (int h = 0; h < 2; h++) {
+        if (!(h ? in1 : in0)) continue;
+        const int l = r * kDiffRX + (h ? c1 : c0);
+        const float sum = ((src[l - 1] + src[l + 1]) + src[l - kDiffRX]) + src[l + kDiffRX];
         dst[l] = rnd<T>((src[l] + a * sum) / den);
       }
     }
     __syncthreads();
   }
   const float* res = buf[rounds & 1];
-  for (int r = kDiffHalo + wave; r < kDiffHalo + kDiffT; r += 4) {
+  for (int r = kDiffHalo + wave; r < kDiffHalo + kDiffTY; r += 8) {
     const int j = y0 + r;
     if (j >= sp.H) break;
-    const int i = x0 + kDiffHalo + lane;
-    if (i >= sp.W) continue;
-    const long k = (long)j * sp.P + i;
-    stf(u_out, k, res[r * kDiffR + kDiffHalo + lane]);
-    if (copy_v) v_out[k] = v_in[k];
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      const int c = kDiffHalo + lane + 64 * h, i = x0 + c;
+      if (c >= kDiffHalo + kDiffTX || i >= sp.W) continue;
+      const long k = (long)j * sp.P + i;
+      stf(u_out, k, res[r * kDiffRX + c]);
+      if (copy_v) v_out[k] = v_in[k];
+    }
   }
 }
 
@@ -247,13 +258,13 @@ __global__ void __launch_bounds__(256)
 template <typename T>
 void enqueue_diffusion(const StepParams& sp, const FieldPtrs& f, int* uv, hipStream_t s) {
   if (sp.n <= 0) return;
-  const int ntx = (sp.W + kDiffT - 1) / kDiffT, nty = (sp.H + kDiffT - 1) / kDiffT;
+  const int ntx = (sp.W + kDiffTX - 1) / kDiffTX, nty = (sp.H + kDiffTY - 1) / kDiffTY;
   const int launches = (sp.n + kDiffHalo - 1) / kDiffHalo;
   int cur = *uv;
   for (int q = 0; q < launches; q++) {
     const int rounds = q + 1 < launches ? kDiffHalo : sp.n - kDiffHalo * (launches - 1);
     const int copy_v = (q + 1 == launches) && (launches & 1);
-    hipLaunchKernelGGL(k_diffuse<T>, dim3(ntx * nty), dim3(256), 0, s, sp, rounds,
+    hipLaunchKernelGGL(k_diffuse<T>, dim3(ntx * nty), dim3(512), 0, s, sp, rounds,
                        (const T*)f.u[cur], (T*)f.u[1 - cur], (const T*)f.v[cur],
                        (T*)f.v[1 - cur], copy_v, ntx);
     cur = 1 - cur;
