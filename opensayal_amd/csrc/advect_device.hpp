@@ -4,8 +4,9 @@
 //
 // A tile is 64 columns by TY rows, run by 64 * NY threads (thread t owns
 // column t % 64 and rows t / 64 + NY * m).  The advections first stage the
-// tile plus a 2-cell halo of every field they sample in LDS with coalesced
-// loads; a back-trace that leaves the staged region reads HBM directly.
+// tile plus a kAdvHalo-cell halo of every field they sample in LDS with
+// coalesced loads; a back-trace that leaves the staged region reads HBM
+// directly.
 #pragma once
 
 #include "fluid_device.hpp"
@@ -16,7 +17,10 @@
 
 namespace osy {
 
-constexpr int kAdvTX = 64, kAdvHalo = 2, kAdvRX = kAdvTX + 2 * kAdvHalo;
+// The 4-cell halo keeps back-traces of up to about 3 cells inside the
+// staged region: a wind-tunnel flow under gravity reaches |v| * d_t of 3-4
+// cells, and a wavefront with one lane outside takes the HBM path.
+constexpr int kAdvTX = 64, kAdvHalo = 4, kAdvRX = kAdvTX + 2 * kAdvHalo;
 
 template <int TY>
 struct AdvShared {
@@ -39,6 +43,19 @@ struct AdvRegion {
            (unsigned)(j - y0 - 1) < (unsigned)(AdvShared<TY>::RY - 2);
   }
 };
+
+// Calls fill(rr, cc) once for every staged cell (rr < RY, cc < kAdvRX).
+// Wavefront w takes rows w, w + NY, ... with lane = column for the first 64
+// columns (the row is wave-uniform, so row-only work runs on the scalar
+// unit); then all threads share the remaining kAdvRX - 64 columns.
+template <int TY, int NY, class Fill>
+__device__ __attribute__((always_inline)) inline void stage_region(int tid, const Fill& fill) {
+  constexpr int RY = AdvShared<TY>::RY, EX = kAdvRX - 64;
+  const int lane = tid & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+  for (int rr = wv; rr < RY; rr += NY) fill(rr, lane);
+  for (int q = tid; q < RY * EX; q += 64 * NY) fill(q / EX, 64 + q % EX);
+}
 
 // Cell of a sample position (the index computation the samplers start with).
 template <bool CS1>
@@ -95,14 +112,14 @@ __device__ __attribute__((always_inline)) inline void advect_velocity_tile(const
   // stays live across a persistent kernel's loop)
   int tid = (int)threadIdx.x;
   asm volatile("" : "+v"(tid));
-  for (int q = tid; q < S::N; q += 64 * NY) {
-    const int rr = q / kAdvRX, cc = q - rr * kAdvRX;
+  stage_region<TY, NY>(tid, [&](int rr, int cc) {
+    const int q = rr * kAdvRX + cc;
     const int i = R.x0 + cc, j = R.y0 + rr;
     const bool in = g.in(i, j);
     su[q] = in ? u_extrapolated(u, g, i, j) : 0.0f;
     sv[q] = in ? v_extrapolated(v, g, i, j) : 0.0f;
     sf[q] = in ? g.flags[g.at(i, j)] : (uint8_t)0;
-  }
+  });
   __syncthreads();
   auto valid = [&](int i, int j) -> bool {
     return R.in(i, j) ? (sf[R.at(i, j)] & CF_FLUID) != 0 : g.valid(i, j);
@@ -190,8 +207,8 @@ __device__ __attribute__((always_inline)) inline void advect_smoke_tile(const St
   // stays live across a persistent kernel's loop)
   int tid = (int)threadIdx.x;
   asm volatile("" : "+v"(tid));
-  for (int q = tid; q < S::N; q += 64 * NY) {
-    const int rr = q / kAdvRX, cc = q - rr * kAdvRX;
+  stage_region<TY, NY>(tid, [&](int rr, int cc) {
+    const int q = rr * kAdvRX + cc;
     const int i = R.x0 + cc, j = R.y0 + rr;
     const bool in = g.in(i, j);
     const long k = in ? g.at(i, j) : 0;
@@ -199,7 +216,7 @@ __device__ __attribute__((always_inline)) inline void advect_smoke_tile(const St
     sv[q] = in ? ldf(v, k) : 0.0f;
     ss[q] = in ? rnd<T>(force_smoke(sp, i, j, ldf(smoke, k))) : 0.0f;
     sf[q] = in ? g.flags[k] : (uint8_t)0;
-  }
+  });
   __syncthreads();
   auto valid = [&](int i, int j) -> bool {
     return R.in(i, j) ? (sf[R.at(i, j)] & CF_FLUID) != 0 : g.valid(i, j);
@@ -219,8 +236,14 @@ __device__ __attribute__((always_inline)) inline void advect_smoke_tile(const St
     const float cx = centre<CS1>(i, cs), cy = centre<CS1>(j, cs);
     // the velocity at a cell centre samples cells within one of (i, j),
     // which are staged
-    const float vx = lds_sample_u<CS1, kAdvRX>(cs, cx, cy, su, sf, R.x0, R.y0);
-    const float vy = lds_sample_v<CS1, kAdvRX>(cs, cx, cy, sv, sf, R.x0, R.y0);
+    float vx, vy;
+    if constexpr (CS1) {
+      vx = lds_centre_u<kAdvRX>(su, sf, R.at(i, j));
+      vy = lds_centre_v<kAdvRX>(sv, sf, R.at(i, j));
+    } else {
+      vx = lds_sample_u<CS1, kAdvRX>(cs, cx, cy, su, sf, R.x0, R.y0);
+      vy = lds_sample_v<CS1, kAdvRX>(cs, cx, cy, sv, sf, R.x0, R.y0);
+    }
     const float x = cx - vx * sp.d_t;
     const float y = cy - vy * sp.d_t;
     float s;

@@ -352,6 +352,31 @@ __device__ inline float lds_sample_v(int cs, float x, float y, const float* a,
   return (f[b] & CF_FLUID) ? acc : 0.0f;
 }
 
+// lds_sample_u / lds_sample_v at the centre of staged cell b with cell size
+// 1.  There in_x = in_y = 0.5: for u the weights are w_x = 1/2, w_y = 1 (the
+// lower row pair), for v w_x = 1 (the right column pair), w_y = 1/2, so two
+// terms are 1/2 * value and two are 0 * value (+-0, or NaN for an infinite
+// value); they are added in the samplers' order, which gives the samplers'
+// results bit for bit without computing the weights.
+template <int RX>
+__device__ inline float lds_centre_u(const float* a, const uint8_t* f, int b) {
+  const int bo = b - RX;
+  float acc = 0.0f + 0.5f * a[b];
+  acc = add_if(acc, f[b + 1] & CF_FLUID, 0.5f * a[b + 1]);
+  acc = add_if(acc, f[bo] & CF_FLUID, 0.0f * a[bo]);
+  acc = add_if(acc, f[bo + 1] & CF_FLUID, 0.0f * a[bo + 1]);
+  return (f[b] & CF_FLUID) ? acc : 0.0f;
+}
+
+template <int RX>
+__device__ inline float lds_centre_v(const float* a, const uint8_t* f, int b) {
+  float acc = 0.0f + 0.5f * a[b];
+  acc = add_if(acc, f[b + RX] & CF_FLUID, 0.5f * a[b + RX]);
+  acc = add_if(acc, f[b + 1 + RX] & CF_FLUID, 0.0f * a[b + 1 + RX]);
+  acc = add_if(acc, f[b + 1] & CF_FLUID, 0.0f * a[b + 1]);
+  return (f[b] & CF_FLUID) ? acc : 0.0f;
+}
+
 template <bool CS1, class Valid, class Val>
 __device__ inline float sample_smoke_bf(int cs, float x, float y, const Valid& valid,
                                         const Val& val) {

@@ -9,7 +9,7 @@
 // stores).  Both advections write into the other half of a ping-pong pair
 // instead of the reference's buffer-then-copy-back (src/fluid.cu:614-617,
 // 569-571): same values, half the traffic.  The advections stage a tile of
-// the fields plus a 2-cell halo in LDS with coalesced loads; a back-trace
+// the fields plus a 4-cell halo in LDS with coalesced loads; a back-trace
 // that leaves the staged region reads HBM directly.
 #include "advect_device.hpp"
 #include "fluid_device.hpp"
@@ -158,8 +158,8 @@ __global__ void __launch_bounds__(256) k_minmax_final(const float* part, int npa
   }
 }
 
-// ---- P6-P8: the advections over 64 x 16 tiles, 256 threads ---------------
-constexpr int kAdvTY = 16, kAdvNY = 4;
+// ---- P6-P8: the advections over 64 x 32 tiles, 512 threads ---------------
+constexpr int kAdvTY = 32, kAdvNY = 8;
 
 // 1-D grids of ntx x nty tiles in the XCD-aware order (xcd_tile).
 template <typename T, bool CS1>
