@@ -7,9 +7,12 @@
 // into tiles, and every (stage, tile) pair is a work item.  Items are listed
 // stage level by stage level across simulations, so every item's
 // dependencies precede it in the list.  Workgroups claim items in list order
-// from an atomic counter and wait (bounded) for the previous stage of the
-// same simulation to complete before working, so the kernel cannot deadlock
-// whatever number of workgroups is resident.
+// from an atomic counter and wait (bounded) for their dependencies before
+// working, so the kernel cannot deadlock whatever number of workgroups is
+// resident.  A projection item depends only on the tiles of the previous
+// projection (or forces) stage that its region overlaps, read or written;
+// an advection item, whose back-traces may read anywhere in the grid, waits
+// for the whole previous stage.
 #pragma once
 
 #include "kernels.hpp"
@@ -53,16 +56,19 @@ struct BatchStage {
   int dep;     // stage that must be complete first, -1 for none
   int dep_tiles;
   int ntx;     // tiles per row of this stage's tiling
+  int tile_dep;  // 1: wait only for the item's own list of predecessor tiles
 };
 
 struct BatchArgs {
   const BatchSim* sims;
   const BatchStage* stages;
-  const uint2* items;  // (stage, tile)
+  const uint4* items;  // (stage, tile, first dependency, dependency count)
+  const unsigned* deps;  // item indices the tile-level waits read in tdone
   int nitems;
   int grid;                  // workgroups launched
   unsigned long long* head;  // claim counter; each launch advances it by nitems + grid
   unsigned long long* done;  // per stage: tiles completed since the last rebuild
+  unsigned long long* tdone;  // per item: launches that completed it since the last rebuild
   int* error;                // set when a dependency wait exceeds spin_ticks
   long long spin_ticks;      // bound of one dependency wait, wall-clock ticks
   int tw, th, k;             // register-tile projection shape

@@ -124,7 +124,8 @@ int rebuild(osy_batch* b) {
   struct Tables {
     std::vector<BatchSim> sims;
     std::vector<BatchStage> stages;
-    std::vector<uint2> items;
+    std::vector<uint4> items;
+    std::vector<unsigned> deps;
   };
   std::vector<Tables> tabs(b->groups.size());
   size_t bytes = 256;  // error word
@@ -161,10 +162,12 @@ int rebuild(osy_batch* b) {
     // stages and items level by level: a stage's dependency (the previous
     // stage of its simulation) is on the level before, so it precedes it
     std::vector<int> prev(g.member.size(), -1), prev_tiles(g.member.size(), 0);
+    std::vector<int> prev_first(g.member.size(), 0), prev_kind(g.member.size(), -1);
     for (size_t L = 0; L < levels; L++) {
       for (size_t si = 0; si < chains.size(); si++) {
         if (L >= chains[si].size()) continue;
         const StagePlan& p = chains[si][L];
+        const StepParams& sp = t.sims[si].sp;
         BatchStage st{};
         st.sim = (int)si;
         st.kind = p.kind;
@@ -175,17 +178,41 @@ int rebuild(osy_batch* b) {
         st.dep = prev[si];
         st.dep_tiles = prev_tiles[si];
         st.ntx = p.ntx;
+        // a projection launch after a projection or forces stage waits only
+        // for the predecessor tiles within 2k of its owned tile: those wrote
+        // what its region reads, or read what its tile writes (the pressure
+        // sums included, which stay on the owned tile)
+        st.tile_dep = p.kind == BK_PROJ && (prev_kind[si] == BK_PROJ || prev_kind[si] == BK_FORCES);
+        const int first = (int)t.items.size();
         const int id = (int)t.stages.size();
         t.stages.push_back(st);
-        for (int tile = 0; tile < p.ntiles; tile++) t.items.push_back(make_uint2(id, tile));
+        const int ptw = prev_kind[si] == BK_PROJ ? tw : kBatchAdvTX;
+        const int pth = prev_kind[si] == BK_PROJ ? th : kBatchAdvTY;
+        const int pntx = ceil_div(sp.W, ptw), pnty = ceil_div(sp.H, pth);
+        for (int tile = 0; tile < p.ntiles; tile++) {
+          const unsigned d0 = (unsigned)t.deps.size();
+          if (st.tile_dep) {
+            const int tx = tile % p.ntx, ty = tile / p.ntx;
+            const int xl = std::max(0, tx * tw - 2 * k), xh = std::min(sp.W, tx * tw + tw + 2 * k);
+            const int yl = std::max(0, ty * th - 2 * k), yh = std::min(sp.H, ty * th + th + 2 * k);
+            for (int by = yl / pth; by <= std::min(pnty - 1, (yh - 1) / pth); by++)
+              for (int bx = xl / ptw; bx <= std::min(pntx - 1, (xh - 1) / ptw); bx++)
+                t.deps.push_back((unsigned)(prev_first[si] + by * pntx + bx));
+          }
+          t.items.push_back(make_uint4((unsigned)id, (unsigned)tile, d0,
+                                       (unsigned)t.deps.size() - d0));
+        }
         prev[si] = id;
         prev_tiles[si] = p.ntiles;
+        prev_first[si] = first;
+        prev_kind[si] = p.kind;
       }
     }
     bytes += align256(t.sims.size() * sizeof(BatchSim)) +
              align256(t.stages.size() * sizeof(BatchStage)) +
-             align256(t.items.size() * sizeof(uint2)) + align256(8) +
-             align256(t.stages.size() * 8);
+             align256(t.items.size() * sizeof(uint4)) +
+             align256(std::max<size_t>(1, t.deps.size()) * sizeof(unsigned)) + align256(8) +
+             align256(t.stages.size() * 8) + align256(t.items.size() * 8);
   }
 
   hipError_t e = hipMalloc(&b->dev, bytes);
@@ -209,14 +236,21 @@ int rebuild(osy_batch* b) {
     OSY_HIP_CHECK(hipMemcpyAsync(p, t.stages.data(), t.stages.size() * sizeof(BatchStage),
                                  hipMemcpyHostToDevice, b->stream));
     p += align256(t.stages.size() * sizeof(BatchStage));
-    a.items = (const uint2*)p;
-    OSY_HIP_CHECK(hipMemcpyAsync(p, t.items.data(), t.items.size() * sizeof(uint2),
+    a.items = (const uint4*)p;
+    OSY_HIP_CHECK(hipMemcpyAsync(p, t.items.data(), t.items.size() * sizeof(uint4),
                                  hipMemcpyHostToDevice, b->stream));
-    p += align256(t.items.size() * sizeof(uint2));
+    p += align256(t.items.size() * sizeof(uint4));
+    a.deps = (const unsigned*)p;
+    if (!t.deps.empty())
+      OSY_HIP_CHECK(hipMemcpyAsync(p, t.deps.data(), t.deps.size() * sizeof(unsigned),
+                                   hipMemcpyHostToDevice, b->stream));
+    p += align256(std::max<size_t>(1, t.deps.size()) * sizeof(unsigned));
     a.head = (unsigned long long*)p;
     p += align256(8);
     a.done = (unsigned long long*)p;
     p += align256(t.stages.size() * 8);
+    a.tdone = (unsigned long long*)p;
+    p += align256(t.items.size() * 8);
     a.nitems = (int)t.items.size();
     a.error = b->error;
     a.spin_ticks = b->spin_ticks;

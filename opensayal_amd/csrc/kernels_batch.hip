@@ -2,9 +2,10 @@
 // simulations (the per-simulation step is Fluid::update, src/fluid.cu:770-795).
 //
 // Every workgroup loops: claim the next work item from the batch's atomic
-// counter, wait until the previous stage of the same simulation is complete
-// (a per-stage counter of finished tiles, read with agent-scope acquire),
-// run the stage's body on the item's tile with the same device code as the
+// counter, wait for its dependencies (the overlapping tiles of the previous
+// projection stage, per-item counters; or the whole previous stage, a
+// per-stage counter of finished tiles; read with agent-scope acquire), run
+// the stage's body on the item's tile with the same device code as the
 // per-grid kernels, publish completion with an agent-scope release.  Items
 // are ordered so that dependencies come first; a workgroup only ever waits
 // for items already claimed by running workgroups, so the launch drains
@@ -48,6 +49,25 @@ __device__ inline bool wait_count(const unsigned long long* counter, unsigned lo
       __builtin_amdgcn_s_sleep(2);
       if (__hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= target)
         break;
+      if (__hip_atomic_load(error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return false;
+      if (wall_clock64() - t0 > spin_ticks) return false;
+    }
+  }
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  return true;
+}
+
+// Thread 0: wait until every counter named by the item's dependency list
+// has reached `target` (relaxed polls, one bound for the whole list), then
+// one agent-scope acquire.
+__device__ inline bool wait_tiles(const unsigned long long* tdone, const unsigned* deps, int n,
+                                  unsigned long long target, const int* error,
+                                  long long spin_ticks) {
+  const long long t0 = wall_clock64();
+  for (int d = 0; d < n; d++) {
+    const unsigned long long* c = tdone + deps[d];
+    while (__hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+      __builtin_amdgcn_s_sleep(2);
       if (__hip_atomic_load(error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return false;
       if (wall_clock64() - t0 > spin_ticks) return false;
     }
@@ -129,13 +149,15 @@ __global__ void __launch_bounds__(kBatchThreads, OCC2 ? 4 : 1) k_batch_step(Batc
     const unsigned long long epoch = raw / per;
     const int idx = (int)(raw - epoch * per);
     if (idx >= A.nitems) break;
-    const uint2 it = A.items[idx];
+    const uint4 it = A.items[idx];
     const int sid = __builtin_amdgcn_readfirstlane((int)it.x);
     const int tile = __builtin_amdgcn_readfirstlane((int)it.y);
     const BatchStage st = A.stages[sid];
     if (threadIdx.x == 0) {
       bool ok = true;
-      if (st.dep >= 0)
+      if (st.tile_dep)
+        ok = wait_tiles(A.tdone, A.deps + it.z, (int)it.w, epoch + 1, A.error, A.spin_ticks);
+      else if (st.dep >= 0)
         ok = wait_count(A.done + st.dep, (epoch + 1) * (unsigned long long)st.dep_tiles,
                         A.error, A.spin_ticks);
       if (!ok) __hip_atomic_store(A.error, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -155,7 +177,9 @@ __global__ void __launch_bounds__(kBatchThreads, OCC2 ? 4 : 1) k_batch_step(Batc
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     if (threadIdx.x == 0) {
-      __hip_atomic_fetch_add(A.done + sid, 1ULL, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+      __hip_atomic_fetch_add(A.done + sid, 1ULL, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_fetch_add(A.tdone + idx, 1ULL, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       s_claim = __hip_atomic_fetch_add(A.head, 1ULL, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     __syncthreads();
