@@ -170,7 +170,7 @@ def batch_leg(specs, steps: int, warmup: int, plain_steps: int = 3, autotune: bo
     }
 
 
-def diffusion_leg(seed: int, steps: int, warmup: int) -> dict:
+def diffusion_leg(seed: int, steps: int, warmup: int, autotune: bool = True) -> dict:
     """The same 1920x1080 step with the reference's default fluid.viscosity
     (0.001): the forces pass, n = 50 Jacobi diffusion rounds (9 launches) and
     then the plain step's projection and advections."""
@@ -183,7 +183,8 @@ def diffusion_leg(seed: int, steps: int, warmup: int) -> dict:
     fl.set_field("vel_y", v)
     fl.set_field("smoke", s)
     stream = Stream()
-    fl.autotune()
+    if autotune:
+        fl.autotune()
     fl.step_async(warmup, stream=stream)
     a, b = Event(), Event()
     a.record(stream)
@@ -408,7 +409,8 @@ def main() -> int:
         out["frame_export"] = view_leg(fl, args.view_reps)
     fl.close()
     if not args.no_diffusion and args.workload == "full":
-        out["with_diffusion"] = diffusion_leg(args.seed, args.steps // 4, args.warmup)
+        out["with_diffusion"] = diffusion_leg(args.seed, args.steps // 4, args.warmup,
+                                              autotune=not args.no_autotune)
     if not args.no_batch and args.workload == "full":
         # 256 uneven grids (128..1024 wide, n from {20, 50, 100}) and 256
         # small ones (64..384 wide), each as one batch against the per-grid path
