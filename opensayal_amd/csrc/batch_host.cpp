@@ -185,10 +185,14 @@ int rebuild(osy_batch* b) {
         st.tile_dep = p.kind == BK_PROJ && (prev_kind[si] == BK_PROJ || prev_kind[si] == BK_FORCES);
         const int first = (int)t.items.size();
         const int id = (int)t.stages.size();
-        t.stages.push_back(st);
         const int ptw = prev_kind[si] == BK_PROJ ? tw : kBatchAdvTX;
         const int pth = prev_kind[si] == BK_PROJ ? th : kBatchAdvTY;
         const int pntx = ceil_div(sp.W, ptw), pnty = ceil_div(sp.H, pth);
+        // one lane of a wavefront polls each dependency: lists longer than
+        // 64 (very flat tiles) fall back to waiting for the whole stage
+        const int span_x = ceil_div(tw + 4 * k, ptw) + 1, span_y = ceil_div(th + 4 * k, pth) + 1;
+        if (span_x * span_y > 64) st.tile_dep = 0;
+        t.stages.push_back(st);
         for (int tile = 0; tile < p.ntiles; tile++) {
           const unsigned d0 = (unsigned)t.deps.size();
           if (st.tile_dep) {

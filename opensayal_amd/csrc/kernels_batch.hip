@@ -36,7 +36,8 @@ __device__ inline unsigned long long uniform_u64(unsigned long long x) {
   return ((unsigned long long)hi << 32) | lo;
 }
 
-// Thread 0: wait until *counter >= target, polling with relaxed loads (an
+// Wavefront 0 (every lane reads the same counter): wait until *counter >=
+// target, polling with relaxed loads (an
 // acquire per poll would invalidate the L2 each time), then one agent-scope
 // acquire.  Returns false on timeout, or at once when an earlier wait of the
 // launch timed out (so a failed launch drains in about one bound rather than
@@ -57,20 +58,23 @@ __device__ inline bool wait_count(const unsigned long long* counter, unsigned lo
   return true;
 }
 
-// Thread 0: wait until every counter named by the item's dependency list
-// has reached `target` (relaxed polls, one bound for the whole list), then
-// one agent-scope acquire.
+// Wavefront 0: wait until every counter named by the item's dependency list
+// (at most 64, one per lane) has reached `target`.  All lanes poll at once
+// with relaxed loads, so a poll costs one memory round trip whatever the
+// list length; then one agent-scope acquire.  One bound for the whole wait.
 __device__ inline bool wait_tiles(const unsigned long long* tdone, const unsigned* deps, int n,
                                   unsigned long long target, const int* error,
                                   long long spin_ticks) {
+  const int lane = (int)(threadIdx.x & 63);
+  const unsigned long long* c = tdone + (lane < n ? deps[lane] : 0u);
   const long long t0 = wall_clock64();
-  for (int d = 0; d < n; d++) {
-    const unsigned long long* c = tdone + deps[d];
-    while (__hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-      __builtin_amdgcn_s_sleep(2);
-      if (__hip_atomic_load(error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return false;
-      if (wall_clock64() - t0 > spin_ticks) return false;
-    }
+  for (;;) {
+    const bool ready =
+        lane >= n || __hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= target;
+    if (__all(ready)) break;
+    if (__hip_atomic_load(error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return false;
+    if (wall_clock64() - t0 > spin_ticks) return false;
+    __builtin_amdgcn_s_sleep(2);
   }
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   return true;
@@ -153,15 +157,17 @@ __global__ void __launch_bounds__(kBatchThreads, OCC2 ? 4 : 1) k_batch_step(Batc
     const int sid = __builtin_amdgcn_readfirstlane((int)it.x);
     const int tile = __builtin_amdgcn_readfirstlane((int)it.y);
     const BatchStage st = A.stages[sid];
-    if (threadIdx.x == 0) {
+    if (threadIdx.x < 64) {  // wavefront 0 waits; thread 0 reports
       bool ok = true;
       if (st.tile_dep)
         ok = wait_tiles(A.tdone, A.deps + it.z, (int)it.w, epoch + 1, A.error, A.spin_ticks);
       else if (st.dep >= 0)
         ok = wait_count(A.done + st.dep, (epoch + 1) * (unsigned long long)st.dep_tiles,
                         A.error, A.spin_ticks);
-      if (!ok) __hip_atomic_store(A.error, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      s_ok = ok ? 1 : 0;
+      if (threadIdx.x == 0) {
+        if (!ok) __hip_atomic_store(A.error, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        s_ok = ok ? 1 : 0;
+      }
     }
     // thread 0's acquire invalidated the caches this workgroup reads
     // through (its CU's L1 and its XCD's L2); the barrier orders every
