@@ -198,6 +198,44 @@ def diffusion_leg(seed: int, steps: int, warmup: int, autotune: bool = True) -> 
             "unit": "Gcell-steps/s"}
 
 
+def precision_leg(seed: int, steps: int, warmup: int, autotune: bool = True) -> dict:
+    """The mixed-precision configuration, 3840x2160, n = 100, pressure on
+    (FP32 pressure), stepped with u, v, smoke stored as FP32, FP16 and BF16.
+    Algorithmic bytes per cell-step: 46 + 25n in FP32, 26 + 17n with 16-bit
+    storage (forces, the pressure zeroing, per sweep u and v read and written
+    plus the mask and the pressure read and written, both advections)."""
+    from opensayal_amd import Event, Fluid, Stream, workloads as wl
+
+    out = {}
+    for storage in ("f32", "f16", "bf16"):
+        cfg = wl.mixed_precision(storage)
+        n = cfg.projection_n
+        fl = Fluid(cfg)
+        u, v, s = wl.initial_state(cfg, seed)
+        fl.set_field("vel_x", u)
+        fl.set_field("vel_y", v)
+        fl.set_field("smoke", s)
+        stream = Stream()
+        if autotune:
+            fl.autotune()
+        fl.step_async(warmup, stream=stream)
+        a, b = Event(), Event()
+        a.record(stream)
+        fl.step_async(steps, stream=stream)
+        b.record(stream)
+        stream.synchronize()
+        ms = a.elapsed_ms(b) / steps
+        fl.close()
+        value = cfg.width * cfg.height / (ms / 1e3) / 1e9
+        per_cell = 46 + 25 * n if storage == "f32" else 26 + 17 * n
+        out[storage] = {"ms_per_step": round(ms, 4), "value": round(value, 4),
+                        "alg_bytes_per_cell_step": per_cell,
+                        "step_roofline_frac": round(value * 1e9 * per_cell / 8e12, 4)}
+    out.update({"workload": "3840x2160 n=100 pressure on", "steps": steps,
+                "unit": "Gcell-steps/s"})
+    return out
+
+
 def view_leg(fl, reps: int) -> dict:
     """What GraphicsHandler reads each frame, as the synchronous library
     calls a viewer makes: the RGBA frame (kernel plus the W*H*4-byte copy to
@@ -265,6 +303,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--kernel-reps", type=int, default=20)
     ap.add_argument("--no-view", action="store_true", help="skip the frame export leg")
+    ap.add_argument("--no-precision", action="store_true",
+                    help="skip the 3840x2160 FP32/FP16/BF16 storage leg")
     ap.add_argument("--no-diffusion", action="store_true",
                     help="skip the leg with the default viscosity")
     ap.add_argument("--view-reps", type=int, default=20)
@@ -408,6 +448,9 @@ def main() -> int:
     if not args.no_view:
         out["frame_export"] = view_leg(fl, args.view_reps)
     fl.close()
+    if not args.no_precision and args.workload == "full":
+        out["mixed_precision"] = precision_leg(args.seed, max(4, args.steps // 15), 5,
+                                               autotune=not args.no_autotune)
     if not args.no_diffusion and args.workload == "full":
         out["with_diffusion"] = diffusion_leg(args.seed, args.steps // 4, args.warmup,
                                               autotune=not args.no_autotune)
