@@ -268,8 +268,8 @@ def sweep_shapes(fl, stream, steps: int = 40) -> None:
             for k in (3, 4):
                 shapes.append(dict(projection_kernel=1, tile_width=tw, tile_height=th,
                                    sweeps_per_launch=k))
-    for rows in (4, 8, 16):
-        for k in (1, 2, 3, 4, 5, 6, 8, 10, 12):
+    for rows in (4, 8, 10, 12, 16):
+        for k in (1, 2, 3, 4, 5, 6, 7, 8, 10, 12):
             if 8 * rows - 4 * k >= 1:
                 shapes.append(dict(projection_kernel=2, tile_width=128 - 4 * k,
                                    tile_height=8 * rows - 4 * k, sweeps_per_launch=k))

@@ -495,14 +495,14 @@ TuneKey tune_key(const osy_fluid* fl) {
 }
 
 // Every launch shape the autotuner times: the colour-phase kernel, the
-// register-resident kernel at 4, 8 and 16 rows per wavefront and 1..12
+// register-resident kernel at 4, 8, 10, 12 and 16 rows per wavefront and 1..12
 // sweeps per launch, and a few LDS-tiled shapes.
 std::vector<ProjTuning> tuning_candidates(bool pressure) {
   std::vector<ProjTuning> out;
   ProjTuning c;
   c.kernel = PROJ_COLOUR;
   out.push_back(c);
-  for (int rows : {4, 8, 16})
+  for (int rows : {4, 8, 10, 12, 16})
     for (int k : {1, 2, 3, 4, 5, 6, 7, 8, 10, 12}) {
       ProjTuning r;
       r.kernel = PROJ_REGS;

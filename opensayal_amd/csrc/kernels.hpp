@@ -69,7 +69,9 @@ enum ProjectionKernel : int {
 // Register-resident kernel: 8 wavefronts stacked vertically, each holding
 // 128 cells x kRegRows rows in VGPRs; the region is 128 x 8*kRegRows cells.
 constexpr int kRegWaves = 8;
-inline bool reg_rows_supported(int rows) { return rows == 4 || rows == 8 || rows == 16; }
+inline bool reg_rows_supported(int rows) {
+  return rows == 4 || rows == 8 || rows == 10 || rows == 12 || rows == 16;
+}
 
 // Workgroup of the tiled kernel: 8 wavefronts of 64 lanes.
 constexpr int kTileWaves = 8;

@@ -245,8 +245,10 @@ __global__ void __launch_bounds__(kTileThreads)
   }
 }
 
+// 10 rows per wavefront are held within 128 VGPRs, so two workgroups share
+// a CU as they do at 8 rows (the others take what the compiler allocates).
 template <typename T, bool PRESSURE, int R>
-__global__ void __launch_bounds__(64 * kRegWaves)
+__global__ void __launch_bounds__(64 * kRegWaves, R == 10 ? 4 : 1)
     k_project_regs(StepParams sp, const uint8_t* __restrict__ gates, const T* __restrict__ u_in,
                    const T* __restrict__ v_in, T* __restrict__ u_out, T* __restrict__ v_out,
                    float* __restrict__ p, RegArgs a) {
@@ -305,6 +307,8 @@ void launch_regs(const StepParams& sp, const GridView& g, const FieldPtrs& f, in
   switch (t.reg_rows()) {
     case 4: launch_regs_r<T, PRESSURE, 4>(sp, g, f, cur, t, s); break;
     case 8: launch_regs_r<T, PRESSURE, 8>(sp, g, f, cur, t, s); break;
+    case 10: launch_regs_r<T, PRESSURE, 10>(sp, g, f, cur, t, s); break;
+    case 12: launch_regs_r<T, PRESSURE, 12>(sp, g, f, cur, t, s); break;
     default: launch_regs_r<T, PRESSURE, 16>(sp, g, f, cur, t, s); break;
   }
 }

@@ -227,6 +227,40 @@ def test_projection_shapes_bit_exact(t, pressure):
     assert_bitwise(fl, o, f"steps {TUNINGS[t]}")
 
 
+# register tiles taller than 8 rows per wavefront (pressure off only: the
+# pressure sums would not fit the registers)
+TALL_TUNINGS = [
+    dict(projection_kernel=2, tile_width=104, tile_height=56, sweeps_per_launch=6),  # 10 rows
+    dict(projection_kernel=2, tile_width=108, tile_height=60, sweeps_per_launch=5),  # 10 rows
+    dict(projection_kernel=2, tile_width=100, tile_height=68, sweeps_per_launch=7),  # 12 rows
+    dict(projection_kernel=2, tile_width=108, tile_height=108, sweeps_per_launch=5),  # 16 rows
+]
+
+
+@pytest.mark.parametrize("t", range(len(TALL_TUNINGS)))
+def test_tall_register_tiles_bit_exact(t):
+    """Tall register tiles on a grid several tiles tall and wide, n = 13 (not
+    a multiple of k), and at full size."""
+    cfg = make_config(250, 230, projection_n=13, wind_speed=2.0, g=-5.0, obstacle_center_x=90,
+                      obstacle_center_y=110, obstacle_radius=20.0)
+    fl, o = pair(cfg, seed=14)
+    fl.set_tuning(**TALL_TUNINGS[t])
+    fl.step(3)
+    o.update(steps=3)
+    assert_bitwise(fl, o, f"steps {TALL_TUNINGS[t]}")
+    with pytest.raises(OpenSayalError):
+        Fluid(make_config(64, 36, enable_pressure=1)).set_tuning(**TALL_TUNINGS[t])
+
+
+def test_tall_register_tile_full_size():
+    cfg = wl.plain_full()
+    fl, o = pair(cfg, seed=15)
+    fl.set_tuning(**TALL_TUNINGS[0])
+    fl.step(2)
+    o.update(steps=2)
+    assert_bitwise(fl, o, "1920x1080, 10 rows per wavefront")
+
+
 @pytest.mark.parametrize("t", [1, 4, 8])
 def test_full_size_tiled_steps_bit_exact(t):
     cfg = wl.plain_full()

@@ -219,6 +219,7 @@ def emulate_projection(cfg, o, K, R, mask=True):
     (130, 70, 4, 4, 8, 1),     # pressure on, k = 4
     (60, 25, 3, 3, 4, 0),      # grid smaller than one tile row
     (250, 90, 11, 5, 8, 0),    # the default shape, tiles on both edges
+    (230, 150, 13, 6, 10, 0),  # 10 rows per wavefront
 ])
 def test_regs_emulation_matches_oracle(W, H, n, K, R, pressure, mask):
     cfg = make_config(W, H, projection_n=n, enable_pressure=pressure,
