@@ -26,9 +26,9 @@ enum BatchStageKind : int {
   BK_ADV_SMOKE = 3,  // smoke advection + decay
 };
 
-// Tile of the forces and advection stages in a batch: 64 x 32 cells, one
+// Tile of the forces and advection stages in a batch: 64 x 64 cells, one
 // row group per wavefront.
-constexpr int kBatchAdvTX = 64, kBatchAdvTY = 32;
+constexpr int kBatchAdvTX = 64, kBatchAdvTY = 64;
 constexpr int kBatchThreads = 64 * kRegWaves;  // 512
 
 // Per simulation: parameters, buffers, and the buffer parity rule.  At launch
