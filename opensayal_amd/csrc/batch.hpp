@@ -76,10 +76,11 @@ struct BatchArgs {
 
 // Register-tile rows per wavefront supported by the batch kernel, and its
 // occupancy variants: 1 = registers as the compiler allocates them, 2 = capped
-// for two workgroups per CU (rows 8 only; rows 4 has two anyway).
-inline bool batch_rows_supported(int rows) { return rows == 4 || rows == 8; }
+// for two workgroups per CU (rows 8 and 10; rows 4 has two anyway; rows 10
+// only this way).
+inline bool batch_rows_supported(int rows) { return rows == 4 || rows == 8 || rows == 10; }
 inline bool batch_occupancy_supported(int rows, int occupancy) {
-  return occupancy == 1 || (occupancy == 2 && rows == 8);
+  return (occupancy == 1 && rows != 10) || (occupancy == 2 && (rows == 8 || rows == 10));
 }
 
 // Occupancy-derived workgroup count for the batch kernel of this storage /

@@ -99,7 +99,7 @@ def test_batch_member_matches_oracle():
 
 
 SHAPES = [(4, 1, 1), (4, 3, 1), (4, 7, 1), (8, 2, 1), (8, 5, 1), (8, 9, 1), (8, 12, 1),
-          (8, 3, 2), (8, 6, 2)]
+          (8, 3, 2), (8, 6, 2), (10, 5, 2), (10, 8, 2), (10, 13, 2)]
 
 
 @pytest.mark.parametrize("rows,k,occ", SHAPES)
@@ -164,7 +164,7 @@ def test_autotune_restores_state():
             b.add(fl)
         before = [state(fl) for fl in batched]
         tuning, ms = b.autotune()
-        assert ms > 0 and tuning["rows"] in (4, 8) and tuning["occupancy"] in (1, 2)
+        assert ms > 0 and tuning["rows"] in (4, 8, 10) and tuning["occupancy"] in (1, 2)
         for fl, s in zip(batched, before):
             assert_same(state(fl), s, "after autotune")
         b.step(2)
