@@ -140,10 +140,10 @@ typedef struct osy_tuning {
 
 typedef struct osy_fluid osy_fluid;
 
-/* Projection launch shape of a batch: register-tile rows per wavefront (4 or
- * 8; the tile is (128 - 4k) x (8 * rows - 4k) cells with a 2k-cell halo),
- * sweeps per tile k, and occupancy: 1 = the compiler's register allocation,
- * 2 = registers capped for two workgroups per CU (rows 8 only); 0 means 1. */
+/* Projection launch shape of a batch: register-tile rows per wavefront (8 or
+ * 12; the tile is (128 - 4k) x (8 * rows - 4k) cells with a 2k-cell halo),
+ * sweeps per tile k, and occupancy: 1 = the compiler's register allocation
+ * (rows 8 only), 2 = registers capped for two workgroups per CU; 0 means 1. */
 typedef struct osy_batch_tuning {
   int32_t rows;
   int32_t sweeps_per_tile;

@@ -76,11 +76,10 @@ struct BatchArgs {
 
 // Register-tile rows per wavefront supported by the batch kernel, and its
 // occupancy variants: 1 = registers as the compiler allocates them, 2 = capped
-// for two workgroups per CU (rows 8 and 10; rows 4 has two anyway; rows 10
-// only this way).
-inline bool batch_rows_supported(int rows) { return rows == 4 || rows == 8 || rows == 10; }
+// for two workgroups per CU (rows 8 both ways, rows 12 only capped).
+inline bool batch_rows_supported(int rows) { return rows == 8 || rows == 12; }
 inline bool batch_occupancy_supported(int rows, int occupancy) {
-  return (occupancy == 1 && rows != 10) || (occupancy == 2 && (rows == 8 || rows == 10));
+  return (occupancy == 1 && rows == 8) || occupancy == 2;
 }
 
 // Occupancy-derived workgroup count for the batch kernel of this storage /

@@ -417,8 +417,8 @@ int osy_batch_set_tuning(osy_batch* b, const osy_batch_tuning* t) {
   const int occ = t->occupancy == 0 ? 1 : t->occupancy;
   if (!tuning_ok(t->rows, t->sweeps_per_tile, occ))
     return set_error(OSY_E_INVALID,
-                     "batch tuning needs rows 4, 8 or 10, 1 <= sweeps_per_tile < 2 * rows, "
-                     "and occupancy 1 (rows 4 or 8) or 2 (rows 8 or 10)");
+                     "batch tuning needs rows 8 or 12, 1 <= sweeps_per_tile < 2 * rows, and "
+                     "occupancy 1 (rows 8) or 2");
   int st = sync_batch(b);
   if (st != OSY_OK) return st;
   b->rows = t->rows;
@@ -464,7 +464,7 @@ int osy_batch_autotune(osy_batch* b, osy_batch_tuning* chosen, double* ms) {
   if (e == hipSuccess) e = hipEventCreate(&a);
   if (e == hipSuccess) e = hipEventCreate(&z);
   st = e == hipSuccess ? OSY_OK : set_error(OSY_E_HIP, hipGetErrorString(e));
-  const int variants[4][2] = {{4, 1}, {8, 1}, {8, 2}, {10, 2}};
+  const int variants[3][2] = {{8, 1}, {8, 2}, {12, 2}};
   for (const auto& var : variants) {
     const int rows = var[0], occ = var[1];
     for (int k = 1; k <= 12 && st == OSY_OK; k++) {

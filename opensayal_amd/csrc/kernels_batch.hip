@@ -195,8 +195,7 @@ __global__ void __launch_bounds__(kBatchThreads, OCC2 ? 4 : 1) k_batch_step(Batc
 
 template <typename T, bool PRESSURE>
 const void* batch_kernel(int rows, int occupancy) {
-  if (rows == 4) return (const void*)k_batch_step<T, PRESSURE, 4, false>;
-  if (rows == 10) return (const void*)k_batch_step<T, PRESSURE, 10, true>;
+  if (rows == 12) return (const void*)k_batch_step<T, PRESSURE, 12, true>;
   return occupancy == 2 ? (const void*)k_batch_step<T, PRESSURE, 8, true>
                         : (const void*)k_batch_step<T, PRESSURE, 8, false>;
 }

@@ -26,7 +26,7 @@ def test_persistent_loop_is_a_single_uniform_loop(tmp_path):
     subprocess.run(cmd, check=True, capture_output=True, cwd=tmp_path)
     lines = out.read_text().split("\n")
     starts = [i for i, l in enumerate(lines) if re.match(r"^_Z\w*k_batch_step\w*:", l)]
-    assert len(starts) == 24
+    assert len(starts) == 18
     for a in starts:
         end = next(i for i in range(a, len(lines)) if "s_endpgm" in lines[i])
         body = lines[a:end]

@@ -98,8 +98,8 @@ def test_batch_member_matches_oracle():
         assert np.array_equal(fl.get_field(name).view(np.uint32), getattr(o, attr).view(np.uint32))
 
 
-SHAPES = [(4, 1, 1), (4, 3, 1), (4, 7, 1), (8, 2, 1), (8, 5, 1), (8, 9, 1), (8, 12, 1),
-          (8, 3, 2), (8, 6, 2), (10, 5, 2), (10, 8, 2), (10, 13, 2)]
+SHAPES = [(8, 2, 1), (8, 5, 1), (8, 9, 1), (8, 12, 1), (8, 3, 2), (8, 6, 2), (12, 1, 2),
+          (12, 7, 2), (12, 10, 2), (12, 17, 2)]
 
 
 @pytest.mark.parametrize("rows,k,occ", SHAPES)
@@ -164,7 +164,7 @@ def test_autotune_restores_state():
             b.add(fl)
         before = [state(fl) for fl in batched]
         tuning, ms = b.autotune()
-        assert ms > 0 and tuning["rows"] in (4, 8, 10) and tuning["occupancy"] in (1, 2)
+        assert ms > 0 and tuning["rows"] in (8, 12) and tuning["occupancy"] in (1, 2)
         for fl, s in zip(batched, before):
             assert_same(state(fl), s, "after autotune")
         b.step(2)
@@ -192,7 +192,7 @@ def test_budget_and_state_errors():
         with pytest.raises(OpenSayalError):
             b.set_tuning(rows=16, sweeps_per_tile=2)
         with pytest.raises(OpenSayalError):
-            b.set_tuning(rows=4, sweeps_per_tile=2, occupancy=2)
+            b.set_tuning(rows=12, sweeps_per_tile=2, occupancy=1)
         b.remove(f2)
         b.add(f3)
         b.step(1)
