@@ -281,7 +281,8 @@ int osy_fluid_device_bytes(const osy_config* cfg, uint64_t* bytes);
  * divergence_norm first wait for the batch's queued steps.  Each member
  * steps with its own configuration's time.d_t. */
 
-/* `budget_bytes` bounds the sum of the members' device bytes (0 = none). */
+/* `budget_bytes` bounds the sum of the members' device bytes (0 = none).
+ * The batch starts with the shape rows 12, sweeps_per_tile 10, occupancy 2. */
 int osy_batch_create(uint64_t budget_bytes, osy_batch** out);
 /* Members leave the batch (they stay valid).  NULL is accepted. */
 int osy_batch_destroy(osy_batch* batch);

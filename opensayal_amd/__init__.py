@@ -399,7 +399,7 @@ class Batch:
         check(lib().osy_batch_get_tuning(self._h, ctypes.byref(t)))
         return {"rows": t.rows, "sweeps_per_tile": t.sweeps_per_tile, "occupancy": t.occupancy}
 
-    def set_tuning(self, rows: int = 8, sweeps_per_tile: int = 5, occupancy: int = 1) -> None:
+    def set_tuning(self, rows: int = 12, sweeps_per_tile: int = 10, occupancy: int = 2) -> None:
         t = OsyBatchTuning(rows, sweeps_per_tile, occupancy)
         check(lib().osy_batch_set_tuning(self._h, ctypes.byref(t)))
 

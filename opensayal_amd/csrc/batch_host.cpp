@@ -20,7 +20,9 @@ struct osy_batch {
   hipStream_t stream = nullptr;
   hipStream_t last = nullptr;  // stream of the last enqueued steps
   std::vector<osy_fluid*> members;
-  int rows = 8, k = 5, occupancy = 1;
+  // the shape the batch autotuner picks for the wind-tunnel mixes (256 grids
+  // of 128..1024 and of 64..384 cells wide): 12 rows, 10 sweeps, two per CU
+  int rows = 12, k = 10, occupancy = 2;
   bool dirty = true;
 
   struct Group {
