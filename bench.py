@@ -323,7 +323,8 @@ def main() -> int:
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one GPU only; each simulation runs on a single "
-              "device (multi-GPU sharding is described in DESIGN.md, not built)", file=sys.stderr)
+              "device (multi-GPU sharding is designed, not built): run with --gpus 1",
+              file=sys.stderr)
         return 2
 
     from opensayal_amd import Event, Fluid, Stream, lib as _lib, workloads as wl

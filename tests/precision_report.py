@@ -1,7 +1,7 @@
 """Precision-mode report (run on demand, not collected by pytest).
 
     python -m tests.precision_report [--steps 200] [--width 3840 --height 2160]
-                                     [--out gpurun_out/precision.json]
+                                     [--out precision_report.json]
 
 The 3840x2160 wind tunnel (obstacle r=72 at the centre, pipe 540, n=100,
 o=1.9, pressure on) is stepped from one seeded state on the GPU with u, v and
@@ -45,7 +45,7 @@ def main() -> int:
     ap.add_argument("--height", type=int, default=2160)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--threads", type=int, default=min(16, os.cpu_count() or 1))
-    ap.add_argument("--out", default="gpurun_out/precision.json")
+    ap.add_argument("--out", default="precision_report.json")
     args = ap.parse_args()
 
     W, H = args.width, args.height
