@@ -6,8 +6,8 @@
 // --compare-performance (inc/logger.hpp:58-95, a frame time in report.log)
 // become a JSON performance record.
 //
-// The program talks to the solver only through the C ABI of
-// libopensayal_amd.so (include/opensayal.h).
+// The program talks to the solver only through the reference-shaped C++
+// wrapper (include/opensayal.hpp) over the C ABI of libopensayal_amd.so.
 #include <cerrno>
 #include <chrono>
 #include <cmath>
@@ -22,7 +22,9 @@
 
 #include <sys/stat.h>
 
-#include "../../include/opensayal.h"
+#include <memory>
+
+#include "../../include/opensayal.hpp"
 #include "json_value.hpp"
 
 namespace {
@@ -177,16 +179,16 @@ bool write_ppm(const std::string& path, const std::vector<uint32_t>& px, int W, 
   return std::fclose(f) == 0 && ok;
 }
 
-int fail(const char* what) {
-  std::fprintf(stderr, "Error: %s: %s\n", what, osy_last_error());
+int fail(const char* what, const std::exception& e) {
+  std::fprintf(stderr, "Error: %s: %s\n", what, e.what());
   return 1;
 }
 
 struct Dumper {
   const Options& o;
-  const osy_config& cfg;
+  const opensayal::Config& cfg;
   const PathLine& pl;
-  osy_fluid* fl;
+  opensayal::Fluid& fl;
 
   std::string name(long step, const char* field, const char* ext) const {
     char b[64];
@@ -197,21 +199,21 @@ struct Dumper {
   int dump(long step, float d_t) const {
     const int W = cfg.width, H = cfg.height;
     const size_t bytes = (size_t)W * H * 4;
-    std::vector<float> buf((size_t)W * H);
-    struct F { int id; const char* name; bool on; };
+    struct F { osy_field id; const char* name; bool on; };
     const F fields[] = {{OSY_FIELD_VEL_X, "vel_x", true},
                         {OSY_FIELD_VEL_Y, "vel_y", true},
                         {OSY_FIELD_SMOKE, "smoke", true},
                         {OSY_FIELD_PRESSURE, "pressure", cfg.enable_pressure != 0}};
     for (const F& f : fields) {
       if (!f.on) continue;
-      if (osy_fluid_get_field(fl, f.id, buf.data(), bytes) != OSY_OK) return fail("get_field");
-      if (!write_npy(name(step, f.name, "npy"), "<f4", {H, W}, buf.data(), bytes))
-        return fail("write");
+      const std::vector<float> buf = fl.field(f.id);
+      if (!write_npy(name(step, f.name, "npy"), "<f4", {H, W}, buf.data(), bytes)) {
+        std::fprintf(stderr, "Error: cannot write %s of step %ld\n", f.name, step);
+        return 1;
+      }
     }
     if (o.frame) {
-      std::vector<uint32_t> px((size_t)W * H);
-      if (osy_fluid_render_rgba(fl, px.data(), bytes) != OSY_OK) return fail("render");
+      const std::vector<uint32_t> px = fl.render_rgba();
       if (!write_npy(name(step, "frame", "npy"), "<u4", {H, W}, px.data(), bytes) ||
           !write_ppm(name(step, "frame", "ppm"), px, W, H)) {
         std::fprintf(stderr, "Error: cannot write the frame of step %ld\n", step);
@@ -220,11 +222,9 @@ struct Dumper {
     }
     if (o.traces || pl.enable) {
       const long th = H / pl.distance, tw = W / pl.distance, L = pl.length;
-      const size_t each = (size_t)th * tw * L * 4;
-      std::vector<int32_t> tx((size_t)th * tw * L), ty(tx.size());
-      if (osy_fluid_traces(fl, pl.distance, pl.length, d_t, tx.data(), ty.data(), each) !=
-          OSY_OK)
-        return fail("traces");
+      std::vector<int32_t> tx, ty;
+      fl.traces(pl.distance, pl.length, d_t, tx, ty);
+      const size_t each = tx.size() * 4;
       if (!write_npy(name(step, "traces_x", "npy"), "<i4", {th, tw, L}, tx.data(), each) ||
           !write_npy(name(step, "traces_y", "npy"), "<i4", {th, tw, L}, ty.data(), each)) {
         std::fprintf(stderr, "Error: cannot write the path lines of step %ld\n", step);
@@ -273,8 +273,12 @@ int main(int argc, char** argv) {
   Options o;
   if (int rc = parse_args(argc, argv, &o)) return rc;
 
-  osy_config cfg;
-  if (osy_config_parse_file(o.config.c_str(), &cfg) != OSY_OK) return fail("config");
+  opensayal::Config cfg;
+  try {
+    cfg = opensayal::ConfigParser(o.config).parse();
+  } catch (const opensayal::Error& e) {
+    return fail("config", e);
+  }
   PathLine pl;
   if (read_path_line(o.config, &pl)) return 1;
   if ((o.traces || pl.enable) && (pl.distance < 1 || pl.length < 1 || pl.length > 65536)) {
@@ -289,82 +293,89 @@ int main(int argc, char** argv) {
     }
   }
 
-  osy_fluid* fl = nullptr;
-  if (osy_fluid_create(&cfg, &fl) != OSY_OK) return fail("create");
+  std::unique_ptr<opensayal::Fluid> fluid;
+  try {
+    fluid = std::make_unique<opensayal::Fluid>(cfg);
+  } catch (const opensayal::Error& e) {
+    return fail("create", e);
+  }
+  opensayal::Fluid& fl = *fluid;
   Dumper dumper{o, cfg, pl, fl};
+  try {
+    // The frame loop of src/main.cu:55-103 without events: the source stays
+    // inactive (no mouse), d_t is sim.time.d_t or, with enable_real_time, the
+    // previous frame's wall time times real_time_multiplier (0 on the first).
+    using clk = std::chrono::steady_clock;
+    const double cells = (double)cfg.width * cfg.height;
+    std::vector<double> frame_us;  // the last 10 frame times (FPS_AVG_SIZE)
+    frame_us.reserve(o.steps);
+    float d_t = cfg.d_t;
+    const auto t0 = clk::now();
+    auto prev = t0;
+    for (long s = 1; s <= o.steps; s++) {
+      const auto now = clk::now();
+      if (s == 1) prev = now;  // the first frame has passed_time 0
+      const double passed_s = std::chrono::duration<double>(now - prev).count();
+      d_t = cfg.enable_real_time ? (float)passed_s * cfg.real_time_multiplier : cfg.d_t;
+      fl.update(opensayal::Source{}, d_t);
+      const auto end = clk::now();
+      frame_us.push_back(std::chrono::duration<double, std::micro>(end - now).count());
+      prev = now;
+      if (o.log_every && s % o.log_every == 0) {
+        const size_t n = frame_us.size() < 10 ? frame_us.size() : 10;
+        double avg = 0;
+        for (size_t k = frame_us.size() - n; k < frame_us.size(); k++) avg += frame_us[k];
+        avg /= (double)n;
+        std::printf("step %ld: FPS: %d, DT: %.0fus\n", s, avg > 0 ? (int)(1e6 / avg) : 0, avg);
+        std::fflush(stdout);
+      }
+      if (o.dump_every && s % o.dump_every == 0 && s != o.steps)
+        if (int rc = dumper.dump(s, d_t)) return rc;
+    }
+    const double wall_s = std::chrono::duration<double>(clk::now() - t0).count();
+    if (!o.dump_dir.empty())
+      if (int rc = dumper.dump(o.steps, d_t)) return rc;
 
-  // The frame loop of src/main.cu:55-103 without events: the source stays
-  // inactive (no mouse), d_t is sim.time.d_t or, with enable_real_time, the
-  // previous frame's wall time times real_time_multiplier (0 on the first).
-  using clk = std::chrono::steady_clock;
-  const double cells = (double)cfg.width * cfg.height;
-  std::vector<double> frame_us;  // the last 10 frame times (FPS_AVG_SIZE)
-  frame_us.reserve(o.steps);
-  float d_t = cfg.d_t;
-  const auto t0 = clk::now();
-  auto prev = t0;
-  for (long s = 1; s <= o.steps; s++) {
-    const auto now = clk::now();
-    if (s == 1) prev = now;  // the first frame has passed_time 0
-    const double passed_s = std::chrono::duration<double>(now - prev).count();
-    d_t = cfg.enable_real_time ? (float)passed_s * cfg.real_time_multiplier : cfg.d_t;
-    if (osy_fluid_update(fl, nullptr, d_t) != OSY_OK) return fail("update");
-    const auto end = clk::now();
-    frame_us.push_back(std::chrono::duration<double, std::micro>(end - now).count());
-    prev = now;
-    if (o.log_every && s % o.log_every == 0) {
-      const size_t n = frame_us.size() < 10 ? frame_us.size() : 10;
-      double avg = 0;
-      for (size_t k = frame_us.size() - n; k < frame_us.size(); k++) avg += frame_us[k];
-      avg /= (double)n;
-      std::printf("step %ld: FPS: %d, DT: %.0fus\n", s, avg > 0 ? (int)(1e6 / avg) : 0, avg);
-      std::fflush(stdout);
+    const size_t n = frame_us.size() < 10 ? frame_us.size() : 10;
+    double avg_frame_us = 0;
+    for (size_t k = frame_us.size() - n; k < frame_us.size(); k++) avg_frame_us += frame_us[k];
+    if (n) avg_frame_us /= (double)n;
+    const double ms_per_step = o.steps ? wall_s * 1e3 / (double)o.steps : 0.0;
+    const double gcell = wall_s > 0 ? cells * (double)o.steps / wall_s / 1e9 : 0.0;
+    osy_fluid_info info;
+    opensayal::check(osy_fluid_get_info(fl.handle(), &info));
+    static const char* kStorage[] = {"f32", "f16", "bf16"};
+    char rec[1024];
+    std::snprintf(rec, sizeof rec,
+                  "{\"config\": \"%s\", \"width\": %d, \"height\": %d, \"projection_n\": %d, "
+                  "\"storage\": \"%s\", \"steps\": %ld, \"real_time\": %s, \"last_d_t\": %.9g, "
+                  "\"wall_s\": %.6f, \"ms_per_step\": %.6f, \"gcell_steps_per_s\": %.6f, "
+                  "\"avg_frame_us\": %.3f, \"fps_avg\": %d, \"device_bytes\": %zu}",
+                  json_escape(o.config).c_str(), cfg.width, cfg.height, cfg.projection_n,
+                  kStorage[info.storage < 3 ? info.storage : 0], o.steps,
+                  cfg.enable_real_time ? "true" : "false", (double)d_t, wall_s, ms_per_step, gcell,
+                  avg_frame_us, avg_frame_us > 0 ? (int)(1e6 / avg_frame_us) : 0,
+                  info.device_bytes);
+    std::printf("%s\n", rec);
+    if (o.save_report) {
+      std::ofstream f(o.report);
+      if (!f) {
+        std::fprintf(stderr, "Error: couldn't open report file %s\n", o.report.c_str());
+        return 1;
+      }
+      f << rec << "\n";
     }
-    if (o.dump_every && s % o.dump_every == 0 && s != o.steps)
-      if (int rc = dumper.dump(s, d_t)) return rc;
-  }
-  const double wall_s = std::chrono::duration<double>(clk::now() - t0).count();
-  if (!o.dump_dir.empty())
-    if (int rc = dumper.dump(o.steps, d_t)) return rc;
-
-  const size_t n = frame_us.size() < 10 ? frame_us.size() : 10;
-  double avg_frame_us = 0;
-  for (size_t k = frame_us.size() - n; k < frame_us.size(); k++) avg_frame_us += frame_us[k];
-  if (n) avg_frame_us /= (double)n;
-  const double ms_per_step = o.steps ? wall_s * 1e3 / (double)o.steps : 0.0;
-  const double gcell = wall_s > 0 ? cells * (double)o.steps / wall_s / 1e9 : 0.0;
-  osy_fluid_info info;
-  osy_fluid_get_info(fl, &info);
-  static const char* kStorage[] = {"f32", "f16", "bf16"};
-  char rec[1024];
-  std::snprintf(rec, sizeof rec,
-                "{\"config\": \"%s\", \"width\": %d, \"height\": %d, \"projection_n\": %d, "
-                "\"storage\": \"%s\", \"steps\": %ld, \"real_time\": %s, \"last_d_t\": %.9g, "
-                "\"wall_s\": %.6f, \"ms_per_step\": %.6f, \"gcell_steps_per_s\": %.6f, "
-                "\"avg_frame_us\": %.3f, \"fps_avg\": %d, \"device_bytes\": %zu}",
-                json_escape(o.config).c_str(), cfg.width, cfg.height, cfg.projection_n,
-                kStorage[info.storage < 3 ? info.storage : 0], o.steps,
-                cfg.enable_real_time ? "true" : "false", (double)d_t, wall_s, ms_per_step, gcell,
-                avg_frame_us, avg_frame_us > 0 ? (int)(1e6 / avg_frame_us) : 0,
-                info.device_bytes);
-  std::printf("%s\n", rec);
-  if (o.save_report) {
-    std::ofstream f(o.report);
-    if (!f) {
-      std::fprintf(stderr, "Error: couldn't open report file %s\n", o.report.c_str());
-      return 1;
+    if (o.compare) {  // Logger::report (inc/logger.hpp:68-86)
+      const double before = previous_frame_us(o.report);
+      if (before <= 0) {
+        std::fprintf(stderr, "Warning: no previous record in %s to compare with\n",
+                     o.report.c_str());
+      } else {
+        std::printf("performance gain: %g%%\n", (before - avg_frame_us) / before * 100.0);
+      }
     }
-    f << rec << "\n";
+    return 0;
+  } catch (const opensayal::Error& e) {
+    return fail("solver", e);
   }
-  if (o.compare) {  // Logger::report (inc/logger.hpp:68-86)
-    const double before = previous_frame_us(o.report);
-    if (before <= 0) {
-      std::fprintf(stderr, "Warning: no previous record in %s to compare with\n",
-                   o.report.c_str());
-    } else {
-      std::printf("performance gain: %g%%\n", (before - avg_frame_us) / before * 100.0);
-    }
-  }
-  osy_fluid_destroy(fl);
-  return 0;
 }
