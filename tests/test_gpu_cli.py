@@ -88,3 +88,65 @@ def test_cli_compare_performance(tmp_path):
     r = run_cli("--steps", "12", "--compare-performance", "--log-every", "4", cwd=tmp_path)
     assert "performance gain:" in r.stdout
     assert r.stdout.count("FPS:") == 3
+
+
+WRAPPER_PROGRAM = r'''
+#include "opensayal.hpp"
+#include <cstdio>
+#include <cstring>
+int main() {
+  auto config = opensayal::ConfigParser().parse();
+  opensayal::Fluid fluid(config);
+  opensayal::Source source;
+  source.active = true;
+  source.smoke = 1.0f;
+  source.velocity = 0.5f;
+  source.position = opensayal::Vector2d<int>(60, 30);
+  for (int k = 0; k < 3; k++) fluid.update(source, config.d_t);
+  const std::vector<float> u = fluid.field(OSY_FIELD_VEL_X);
+  unsigned long long h = 1469598103934665603ull;
+  for (float x : u) {
+    unsigned b;
+    std::memcpy(&b, &x, 4);
+    h = (h ^ b) * 1099511628211ull;
+  }
+  std::printf("%d %d %.9g %.9g %llu\n", fluid.width, fluid.height, fluid.min_pressure,
+              fluid.max_pressure, h);
+  return 0;
+}
+'''
+
+
+def test_wrapper_program_matches_python_api(tmp_path):
+    """A program written against the C++ wrapper (active source, 3 updates)
+    ends in the same state and pressure range as the Python API."""
+    import shutil
+    from pathlib import Path
+
+    root = Path(__file__).resolve().parents[1]
+    (tmp_path / "OpenSayal.conf.json").write_text(json.dumps(CONFIG))
+    src = tmp_path / "app.cpp"
+    src.write_text(WRAPPER_PROGRAM)
+    lib_dir = root / "opensayal_amd"
+    subprocess.run([shutil.which("g++") or "g++", "-std=c++17", f"-I{root / 'include'}", str(src),
+                    f"-L{lib_dir}", "-lopensayal_amd", f"-Wl,-rpath,{lib_dir}", "-o",
+                    str(tmp_path / "app")], check=True)
+    r = subprocess.run([str(tmp_path / "app")], capture_output=True, text=True, timeout=120,
+                       cwd=tmp_path)
+    assert r.returncode == 0, r.stderr
+    w, h, pmin, pmax, digest = r.stdout.split()
+
+    from opensayal_amd import Source
+    cfg = ConfigParser(str(tmp_path / "OpenSayal.conf.json")).parse()
+    fl = Fluid(cfg)
+    s = Source()
+    s.active, s.smoke, s.velocity, s.position_x, s.position_y = 1, 1.0, 0.5, 60, 30
+    for _ in range(3):
+        fl.update(s)
+    want = 1469598103934665603
+    for b in fl.get_field("vel_x").view(np.uint32).ravel():
+        want = ((want ^ int(b)) * 1099511628211) & 0xFFFFFFFFFFFFFFFF
+    assert (int(w), int(h)) == (130, 74)
+    assert int(digest) == want
+    lo, hi = fl.pressure_range
+    assert np.float32(float(pmin)) == np.float32(lo) and np.float32(float(pmax)) == np.float32(hi)
