@@ -45,6 +45,16 @@ def step_bytes_per_cell(n: int) -> int:
     return 42 + 17 * n
 
 
+def alg_bytes_per_cell(cfg) -> int:
+    """Algorithmic bytes per cell-step of one configuration: FP32 storage
+    42 + 17n (46 + 25n with pressure), 16-bit storage 22 + 9n (26 + 17n with
+    the FP32 pressure)."""
+    n = cfg.projection_n
+    if cfg.storage == 0:
+        return 46 + 25 * n if cfg.enable_pressure else 42 + 17 * n
+    return 26 + 17 * n if cfg.enable_pressure else 22 + 9 * n
+
+
 def host_threads() -> int:
     try:
         n = len(os.sched_getaffinity(0))
@@ -155,6 +165,8 @@ def batch_leg(specs, steps: int, warmup: int, plain_steps: int = 3, autotune: bo
     value = cells * steps / (total_ms / 1e3) / 1e9
     plain = cells / (plain_ms / 1e3) / 1e9
     ns = sorted(sp.cfg.projection_n for sp in specs)
+    alg = sum(sp.cfg.width * sp.cfg.height * alg_bytes_per_cell(sp.cfg) for sp in specs)
+    achieved = alg * steps / (total_ms / 1e3) / 1e9
     return {
         "sims": len(specs), "cells_per_step": cells, "steps": steps, "warmup": warmup,
         "n_values": sorted(set(ns)),
@@ -166,6 +178,8 @@ def batch_leg(specs, steps: int, warmup: int, plain_steps: int = 3, autotune: bo
         "plain_sequential": {"value": round(plain, 4), "ms_per_step": round(plain_ms, 4),
                              "steps": plain_steps},
         "speedup_vs_plain": round(value / plain, 3),
+        "step_roofline": {"alg_bytes_per_step": int(alg), "achieved_GBs": round(achieved, 1),
+                          "frac": round(achieved / HBM_PEAK_GBS, 4)},
         "setup_s": round(setup_s, 2),
     }
 
