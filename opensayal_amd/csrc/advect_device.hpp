@@ -44,17 +44,50 @@ struct AdvRegion {
   }
 };
 
-// Calls fill(rr, cc) once for every staged cell (rr < RY, cc < kAdvRX).
+// Stages every cell of the region (rr < RY, cc < kAdvRX): load(rr, cc)
+// reads the cell's raw values from HBM (from in-bounds addresses whatever
+// (rr, cc), with no branch), put(rr, cc, vals) writes them to LDS.
 // Wavefront w takes rows w, w + NY, ... with lane = column for the first 64
 // columns (the row is wave-uniform, so row-only work runs on the scalar
-// unit); then all threads share the remaining kAdvRX - 64 columns.
-template <int TY, int NY, class Fill>
-__device__ __attribute__((always_inline)) inline void stage_region(int tid, const Fill& fill) {
+// unit); then all threads share the remaining kAdvRX - 64 columns.  The
+// loops are unrolled and every load of a thread is issued before its first
+// LDS write, so a wavefront waits for HBM once rather than once per row.
+// Slots past the region load a duplicate of its last cell and put nothing.
+template <int TY, int NY, class Load, class Put>
+__device__ __attribute__((always_inline)) inline void stage_region(int tid, const Load& load,
+                                                                   const Put& put) {
   constexpr int RY = AdvShared<TY>::RY, EX = kAdvRX - 64;
+  constexpr int NM = (RY + NY - 1) / NY, NE = (RY * EX + 64 * NY - 1) / (64 * NY);
+  constexpr bool full_m = RY % NY == 0, full_e = (RY * EX) % (64 * NY) == 0;
   const int lane = tid & 63;
   const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
-  for (int rr = wv; rr < RY; rr += NY) fill(rr, lane);
-  for (int q = tid; q < RY * EX; q += 64 * NY) fill(q / EX, 64 + q % EX);
+  using V = decltype(load(0, 0));
+  V a[NM], b[NE];
+#pragma unroll
+  for (int m = 0; m < NM; m++) a[m] = load(full_m ? wv + NY * m : min(wv + NY * m, RY - 1), lane);
+#pragma unroll
+  for (int e = 0; e < NE; e++) {
+    const int q = full_e ? tid + 64 * NY * e : min(tid + 64 * NY * e, RY * EX - 1);
+    b[e] = load(q / EX, 64 + q % EX);
+  }
+  __builtin_amdgcn_sched_barrier(0);  // keep every load ahead of the first write
+  // the shared columns first: the compiler moves loads whose values only a
+  // guarded write uses into the guarded block, which then comes right after
+  // the row loads
+#pragma unroll
+  for (int e = 0; e < NE; e++) {
+    const int q = tid + 64 * NY * e;
+    if (full_e || q < RY * EX) put(q / EX, 64 + q % EX, b[e]);
+  }
+#pragma unroll
+  for (int m = 0; m < NM; m++)
+    if (full_m || wv + NY * m < RY) put(wv + NY * m, lane, a[m]);
+}
+
+// (i, j) clamped into the grid: the address a staged cell outside the grid
+// loads from (its value is then replaced by 0).
+__device__ inline long clamped_at(const GridView& g, int i, int j) {
+  return g.at(min(max(i, 0), g.W - 1), min(max(j, 0), g.H - 1));
 }
 
 // Cell of a sample position (the index computation the samplers start with).
@@ -112,14 +145,28 @@ __device__ __attribute__((always_inline)) inline void advect_velocity_tile(const
   // stays live across a persistent kernel's loop)
   int tid = (int)threadIdx.x;
   asm volatile("" : "+v"(tid));
-  stage_region<TY, NY>(tid, [&](int rr, int cc) {
-    const int q = rr * kAdvRX + cc;
-    const int i = R.x0 + cc, j = R.y0 + rr;
-    const bool in = g.in(i, j);
-    su[q] = in ? u_extrapolated(u, g, i, j) : 0.0f;
-    sv[q] = in ? v_extrapolated(v, g, i, j) : 0.0f;
-    sf[q] = in ? g.flags[g.at(i, j)] : (uint8_t)0;
-  });
+  // u_extrapolated / v_extrapolated with the reads made unconditional: the
+  // remapped cell of (i, j) clamped into the grid
+  struct Raw {
+    float u, v;
+    uint8_t f;
+  };
+  stage_region<TY, NY>(
+      tid,
+      [&](int rr, int cc) {
+        const int i = min(max(R.x0 + cc, 0), g.W - 1), j = min(max(R.y0 + rr, 0), g.H - 1);
+        const int ju = j == 0 ? 1 : (j == g.H - 1 ? g.H - 2 : j);
+        const int iv = i == 0 ? 1 : (i == g.W - 1 ? g.W - 2 : i);
+        return Raw{ldf(u, g.at(i, ju)), ldf(v, g.at(iv, j)), g.flags[g.at(i, j)]};
+      },
+      [&](int rr, int cc, const Raw& r) {
+        const int q = rr * kAdvRX + cc;
+        const int i = R.x0 + cc, j = R.y0 + rr;
+        const bool in = g.in(i, j);
+        su[q] = in && i != 1 ? r.u : 0.0f;
+        sv[q] = in && j != 1 ? r.v : 0.0f;
+        sf[q] = in ? r.f : (uint8_t)0;
+      });
   __syncthreads();
   auto valid = [&](int i, int j) -> bool {
     return R.in(i, j) ? (sf[R.at(i, j)] & CF_FLUID) != 0 : g.valid(i, j);
@@ -207,16 +254,25 @@ __device__ __attribute__((always_inline)) inline void advect_smoke_tile(const St
   // stays live across a persistent kernel's loop)
   int tid = (int)threadIdx.x;
   asm volatile("" : "+v"(tid));
-  stage_region<TY, NY>(tid, [&](int rr, int cc) {
-    const int q = rr * kAdvRX + cc;
-    const int i = R.x0 + cc, j = R.y0 + rr;
-    const bool in = g.in(i, j);
-    const long k = in ? g.at(i, j) : 0;
-    su[q] = in ? ldf(u, k) : 0.0f;
-    sv[q] = in ? ldf(v, k) : 0.0f;
-    ss[q] = in ? rnd<T>(force_smoke(sp, i, j, ldf(smoke, k))) : 0.0f;
-    sf[q] = in ? g.flags[k] : (uint8_t)0;
-  });
+  struct Raw {
+    float u, v, s;
+    uint8_t f;
+  };
+  stage_region<TY, NY>(
+      tid,
+      [&](int rr, int cc) {
+        const long k = clamped_at(g, R.x0 + cc, R.y0 + rr);
+        return Raw{ldf(u, k), ldf(v, k), ldf(smoke, k), g.flags[k]};
+      },
+      [&](int rr, int cc, const Raw& r) {
+        const int q = rr * kAdvRX + cc;
+        const int i = R.x0 + cc, j = R.y0 + rr;
+        const bool in = g.in(i, j);
+        su[q] = in ? r.u : 0.0f;
+        sv[q] = in ? r.v : 0.0f;
+        ss[q] = in ? rnd<T>(force_smoke(sp, i, j, r.s)) : 0.0f;
+        sf[q] = in ? r.f : (uint8_t)0;
+      });
   __syncthreads();
   auto valid = [&](int i, int j) -> bool {
     return R.in(i, j) ? (sf[R.at(i, j)] & CF_FLUID) != 0 : g.valid(i, j);
