@@ -116,6 +116,22 @@ def test_config_variants_bit_exact(case):
     assert_bitwise(fl, o, case)
 
 
+@pytest.mark.parametrize("shape", [(3, 3), (3, 67), (67, 3), (65, 33), (129, 4)])
+def test_smallest_and_ragged_grids_bit_exact(shape):
+    """Grids of one interior row or column, and grids one cell past a tile
+    (64 wide, 32 tall): the advections' staged halo is mostly outside the
+    grid, and the extrapolated borders are read from rows and columns 1 and
+    W-2 / H-2, which here are the same cells."""
+    W, H = shape
+    cfg = make_config(W, H, wind_speed=1.5, g=-3.0, enable_pressure=1, projection_n=7,
+                      pipe_height=max(1, H // 2), smoke_height=max(1, H // 4),
+                      obstacle_radius=0.0)
+    fl, o = pair(cfg, seed=13)
+    fl.step(5)
+    o.update(steps=5)
+    assert_bitwise(fl, o, f"{W}x{H}")
+
+
 def test_active_source_step_bit_exact():
     cfg = make_config(160, 100, wind_speed=1.0, g=-1.0)
     fl, o = pair(cfg, seed=5)
