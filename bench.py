@@ -232,6 +232,7 @@ def precision_leg(seed: int, steps: int, warmup: int, autotune: bool = True) -> 
         stream = Stream()
         if autotune:
             fl.autotune()
+        shape = fl.tuning
         fl.step_async(warmup, stream=stream)
         a, b = Event(), Event()
         a.record(stream)
@@ -244,7 +245,8 @@ def precision_leg(seed: int, steps: int, warmup: int, autotune: bool = True) -> 
         per_cell = 46 + 25 * n if storage == "f32" else 26 + 17 * n
         out[storage] = {"ms_per_step": round(ms, 4), "value": round(value, 4),
                         "alg_bytes_per_cell_step": per_cell,
-                        "step_roofline_frac": round(value * 1e9 * per_cell / 8e12, 4)}
+                        "step_roofline_frac": round(value * 1e9 * per_cell / 8e12, 4),
+                        "projection_shape": shape}
     out.update({"workload": "3840x2160 n=100 pressure on", "steps": steps,
                 "unit": "Gcell-steps/s"})
     return out

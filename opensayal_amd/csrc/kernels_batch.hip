@@ -97,7 +97,7 @@ __device__ __attribute__((always_inline)) inline void run_item(const BatchStage&
     a.sweeps = st.sweeps;
     a.forces = st.forces;
     const int in = uv ^ (st.q & 1);
-    project_regs_tile<T, PRESSURE, R>(sp, sim.gates, (const T*)sim.u[in], (const T*)sim.v[in],
+    project_regs_tile<T, PRESSURE, R, true>(sp, sim.gates, (const T*)sim.u[in], (const T*)sim.v[in],
                                       (T*)sim.u[in ^ 1], (T*)sim.v[in ^ 1], sim.p, a, tile,
                                       sh.reg);
     return;

@@ -208,6 +208,13 @@ __device__ __attribute__((always_inline)) inline void reg_row(RegTile<R>& t, con
   }
 }
 
+// LDS the register-resident tile exchanges rows through, two slots of each
+// (with one barrier per phase consecutive phases use different slots).
+struct RegShared {
+  float2 vrow[2][kRegWaves][64];  // row 0 (v0, v1) of each wavefront
+  float dsend[2][kRegWaves][64];  // top-row update for the wavefront above
+};
+
 // Rows RHO..R-1 of one phase, unrolled at compile time.
 template <typename T, int R, int PB, bool PRESSURE, int RHO>
 __device__ __attribute__((always_inline)) inline void reg_rows_from(RegTile<R>& t, float vt0,
@@ -224,26 +231,46 @@ __device__ __attribute__((always_inline)) inline void reg_rows_from(RegTile<R>& 
 // wavefront (compile time: phases come in pairs with PB and 1 - PB).  A
 // wavefront none of whose rows can still reach the tile skips the phase;
 // the others run it as straight-line code over registers.
-template <typename T, int R, int PB, bool PRESSURE>
-__device__ __attribute__((always_inline)) inline void reg_phase(RegTile<R>& t, float2 (*vrow)[64],
-                                                                float (*dsend)[64], int lane,
+//
+// (vt0, vt1) is row 0 of the wavefront above as it stands when the phase
+// starts: the top faces of this wavefront's last row.  The last row moves
+// those faces by dtop, which the wavefront above applies to its row 0.
+// Two ways to exchange both through LDS:
+//  * two barriers (ONE_SYNC false): each wavefront publishes its row 0 and
+//    reads (vt0, vt1) after the first; sends dtop and applies the one from
+//    below after the second.
+//  * one barrier (ONE_SYNC true): each wavefront publishes its row 0 after
+//    computing (this phase's own updates, before the one from below) beside
+//    dtop; after the barrier it applies the update from below to its row 0,
+//    and the wavefront below applies the same update, with the same
+//    rounding, to the copy it reads, which is then the row as the wavefront
+//    above holds it.  The slots alternate between phases (slot b), so none
+//    is rewritten before every reader of its previous contents has passed
+//    the next barrier.  The caller publishes and reads the rows as loaded
+//    before the first phase.
+// The two give the same bits; the persistent batch kernel, whose items
+// queue behind each other's barriers, is faster with one.
+template <typename T, int R, int PB, bool PRESSURE, bool ONE_SYNC>
+__device__ __attribute__((always_inline)) inline void reg_phase(RegTile<R>& t, RegShared& sh,
+                                                                float& vt0, float& vt1, int lane,
                                                                 int w, int rbase, int s,
-                                                                int nphase, int K, int tw,
-                                                                int th, float o, float density,
+                                                                int nphase, int K, int th,
+                                                                float o, float density,
                                                                 float csf, float d_t) {
-  (void)tw;
   const int e = nphase - 1 - s;
   const int lo = 2 * K - 1 - e;
   const int rhi = 2 * K + th + e;
   const int rlo = lo - rbase, rhi_rel = rhi - rbase;  // box rows of this wavefront
-  // publish row 0 for the wavefront below, read the row above
-  if (w > 0) vrow[w][lane] = make_float2(t.v0[0], t.v1[0]);
-  __syncthreads();
-  float vt0 = 0.0f, vt1 = 0.0f;
-  if (w < kRegWaves - 1) {
-    const float2 x = vrow[w + 1][lane];
-    vt0 = x.x;
-    vt1 = x.y;
+  const int b = ONE_SYNC ? (s + 1) & 1 : 0;
+  if constexpr (!ONE_SYNC) {
+    if (w > 0) sh.vrow[0][w][lane] = make_float2(t.v0[0], t.v1[0]);
+    __syncthreads();
+    vt0 = vt1 = 0.0f;  // not carried from the previous phase
+    if (w < kRegWaves - 1) {
+      const float2 x = sh.vrow[0][w + 1][lane];
+      vt0 = x.x;
+      vt1 = x.y;
+    }
   }
   float dtop = 0.0f;
   if (rlo < R && rhi_rel > 0) {  // wavefront-uniform: some row is in the box
@@ -254,36 +281,43 @@ __device__ __attribute__((always_inline)) inline void reg_phase(RegTile<R>& t, f
     for (int rho = 0; rho < R; rho++) asm volatile("" : "+v"(t.gw[rho]));
     reg_rows_from<T, R, PB, PRESSURE, 0>(t, vt0, vt1, dtop, o, density, csf, d_t);
   }
-  if (w < kRegWaves - 1) dsend[w][lane] = dtop;
+  if constexpr (ONE_SYNC) {
+    if (w > 0) sh.vrow[b][w][lane] = make_float2(t.v0[0], t.v1[0]);
+  }
+  if (w < kRegWaves - 1) sh.dsend[b][w][lane] = dtop;
   __syncthreads();
   if (w > 0) {
     // the wavefront below moved the top faces of its last row: they are
     // this wavefront's row-0 v of parity 1 - PB
-    const float dd = dsend[w - 1][lane];
+    const float dd = sh.dsend[b][w - 1][lane];
     if constexpr (PB == 0)
       t.v1[0] = rnd<T>(t.v1[0] - dd);
     else
       t.v0[0] = rnd<T>(t.v0[0] - dd);
   }
+  if constexpr (ONE_SYNC) {
+    if (w < kRegWaves - 1) {
+      const float2 x = sh.vrow[b][w + 1][lane];
+      if constexpr (PB == 0) {
+        vt0 = x.x;
+        vt1 = rnd<T>(x.y - dtop);
+      } else {
+        vt0 = rnd<T>(x.x - dtop);
+        vt1 = x.y;
+      }
+    }
+  }
 }
-
-// LDS the register-resident tile exchanges rows through.
-struct RegShared {
-  float2 vrow[kRegWaves][64];  // row 0 (v0, v1) of each wavefront
-  float dsend[kRegWaves][64];  // top-row update for the wavefront above
-};
 
 // One launch of the register-resident projection for tile `tile` (row-major
 // over the grid's ntx tiles per row), run by a workgroup of 64 * kRegWaves
 // threads.
-template <typename T, bool PRESSURE, int R>
+template <typename T, bool PRESSURE, int R, bool ONE_SYNC>
 __device__ __attribute__((always_inline)) inline void project_regs_tile(const StepParams& sp, const uint8_t* __restrict__ gates,
                                          const T* __restrict__ u_in, const T* __restrict__ v_in,
                                          T* __restrict__ u_out, T* __restrict__ v_out,
                                          float* __restrict__ p, const RegArgs& a, int tile,
                                          RegShared& sh) {
-  float2 (*vrow)[64] = sh.vrow;
-  float (*dsend)[64] = sh.dsend;
   int tid = (int)threadIdx.x;
   asm volatile("" : "+v"(tid));  // re-read per tile, not kept live across a persistent loop
   const int lane = tid & 63;
@@ -338,21 +372,31 @@ __device__ __attribute__((always_inline)) inline void project_regs_tile(const St
   const float o = sp.o;
   const float csf = (float)sp.cs;
   const int nphase = 2 * a.sweeps;
+  float vt0 = 0.0f, vt1 = 0.0f;
+  if constexpr (ONE_SYNC) {  // row 0 of the wavefront above, as loaded
+    if (w > 0) sh.vrow[0][w][lane] = make_float2(t.v0[0], t.v1[0]);
+    __syncthreads();
+    if (w < kRegWaves - 1) {
+      const float2 x = sh.vrow[0][w + 1][lane];
+      vt0 = x.x;
+      vt1 = x.y;
+    }
+  }
   // colour 0 (even cells) first; row 0's active parity in a colour-0 phase
   // is (gy0 + rbase) & 1 and flips every phase
   if (((gy0 + rbase) & 1) == 0) {
     for (int s = 0; s < nphase; s += 2) {
-      reg_phase<T, R, 0, PRESSURE>(t, vrow, dsend, lane, w, rbase, s, nphase, K, a.tw, a.th, o,
-                                sp.density, csf, sp.d_t);
-      reg_phase<T, R, 1, PRESSURE>(t, vrow, dsend, lane, w, rbase, s + 1, nphase, K, a.tw, a.th, o,
-                                sp.density, csf, sp.d_t);
+      reg_phase<T, R, 0, PRESSURE, ONE_SYNC>(t, sh, vt0, vt1, lane, w, rbase, s, nphase, K, a.th, o,
+                                   sp.density, csf, sp.d_t);
+      reg_phase<T, R, 1, PRESSURE, ONE_SYNC>(t, sh, vt0, vt1, lane, w, rbase, s + 1, nphase, K, a.th, o,
+                                   sp.density, csf, sp.d_t);
     }
   } else {
     for (int s = 0; s < nphase; s += 2) {
-      reg_phase<T, R, 1, PRESSURE>(t, vrow, dsend, lane, w, rbase, s, nphase, K, a.tw, a.th, o,
-                                sp.density, csf, sp.d_t);
-      reg_phase<T, R, 0, PRESSURE>(t, vrow, dsend, lane, w, rbase, s + 1, nphase, K, a.tw, a.th, o,
-                                sp.density, csf, sp.d_t);
+      reg_phase<T, R, 1, PRESSURE, ONE_SYNC>(t, sh, vt0, vt1, lane, w, rbase, s, nphase, K, a.th, o,
+                                   sp.density, csf, sp.d_t);
+      reg_phase<T, R, 0, PRESSURE, ONE_SYNC>(t, sh, vt0, vt1, lane, w, rbase, s + 1, nphase, K, a.th, o,
+                                   sp.density, csf, sp.d_t);
     }
   }
 
