@@ -195,6 +195,7 @@ __global__ void __launch_bounds__(64 * kAdvNY)
 // 126 of 128 lanes.  u goes from u_in to u_out; with copy_v the launch also
 // moves v to v_out so that u and v stay in the same buffer of their pairs.
 constexpr int kDiffRX = 128, kDiffRY = 64, kDiffHalo = 6;
+static_assert(kDiffRY % 8 == 0, "eight wavefronts take the region's rows in turn");
 constexpr int kDiffTX = kDiffRX - 2 * kDiffHalo, kDiffTY = kDiffRY - 2 * kDiffHalo;
 
 template <typename T>
@@ -204,14 +205,32 @@ __global__ void __launch_bounds__(512)
   __shared__ float buf[2][kDiffRY * kDiffRX];
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int x0 = (tile % ntx) * kDiffTX - kDiffHalo, y0 = (tile / ntx) * kDiffTY - kDiffHalo;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (int r = wave; r < kDiffRY; r += 8) {
-    const int j = y0 + r;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  // stage the region: every load of the thread (from the cell clamped into
+  // the grid, so with no branch) before its first LDS write, so a wavefront
+  // waits for HBM once rather than once per row; cells outside the grid
+  // hold 0
+  constexpr int kRowsPerWave = kDiffRY / 8;
+  float staged[kRowsPerWave][2];
+#pragma unroll
+  for (int m = 0; m < kRowsPerWave; m++) {
+    const int j = min(max(y0 + wave + 8 * m, 0), sp.H - 1);
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      const int i = min(max(x0 + lane + 64 * h, 0), sp.W - 1);
+      staged[m][h] = ldf(u_in, (long)j * sp.P + i);
+    }
+  }
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int m = 0; m < kRowsPerWave; m++) {
+    const int r = wave + 8 * m, j = y0 + r;
 #pragma unroll
     for (int h = 0; h < 2; h++) {
       const int c = lane + 64 * h, i = x0 + c;
       const bool in = (unsigned)i < (unsigned)sp.W && (unsigned)j < (unsigned)sp.H;
-      const float x = in ? ldf(u_in, (long)j * sp.P + i) : 0.0f;
+      const float x = in ? staged[m][h] : 0.0f;
       buf[0][r * kDiffRX + c] = x;
       buf[1][r * kDiffRX + c] = x;
     }
@@ -225,9 +244,10 @@ __global__ void __launch_bounds__(512)
   for (int it = 0; it < rounds; it++) {
     const float* src = buf[it & 1];
     float* dst = buf[(it & 1) ^ 1];
-    for (int r = 1 + wave; r < kDiffRY - 1; r += 8) {
-      const int j = y0 + r;
-      if (j < 1 || j > sp.H - 2) continue;
+#pragma unroll
+    for (int m = 0; m < kRowsPerWave; m++) {
+      const int r = 1 + wave + 8 * m, j = y0 + r;
+      if (r >= kDiffRY - 1 || j < 1 || j > sp.H - 2) continue;
 #pragma unroll
       for (int h = 0; h < 2; h++) {
         if (!(h ? in1 : in0)) continue;
