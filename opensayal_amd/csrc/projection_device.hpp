@@ -334,38 +334,54 @@ __device__ __attribute__((always_inline)) inline void project_regs_tile(const St
 
   RegTile<R> t;
   t.slow = 0;
+  // Every load of the tile first, from the even column clamped into the row
+  // (so a pair stays inside its row of P cells) and the row clamped into the
+  // grid, with no branch: a wavefront waits for HBM once rather than once
+  // per row.  Cells outside the grid are then set to 0.
+  const int gic = min(max(gi, 0), P - 2);
+  uint32_t graw[R];
+#pragma unroll
+  for (int rho = 0; rho < R; rho++) {
+    const long k = (long)min(max(gy0 + rbase + rho, 0), H - 1) * P + gic;
+    load_pair(u_in, k, true, true, t.u0[rho], t.u1[rho]);
+    load_pair(v_in, k, true, true, t.v0[rho], t.v1[rho]);
+    graw[rho] = *reinterpret_cast<const uint16_t*>(gates + k);
+    if constexpr (PRESSURE) {
+      const float2 pp = *reinterpret_cast<const float2*>(p + k);
+      t.s0[rho] = pp.x;
+      t.s1[rho] = pp.y;
+    }
+  }
+  __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
   for (int rho = 0; rho < R; rho++) {
     const int gj = gy0 + rbase + rho;
     const bool row_in = (unsigned)gj < (unsigned)H;
-    const long gk = (long)gj * P + gi;
-    load_pair(u_in, gk, row_in && col0, row_in && col1, t.u0[rho], t.u1[rho]);
-    load_pair(v_in, gk, row_in && col0, row_in && col1, t.v0[rho], t.v1[rho]);
+    const bool in0 = row_in && col0, in1 = row_in && col1;
+    t.u0[rho] = in0 ? t.u0[rho] : 0.0f;
+    t.v0[rho] = in0 ? t.v0[rho] : 0.0f;
+    t.u1[rho] = in1 ? t.u1[rho] : 0.0f;
+    t.v1[rho] = in1 ? t.v1[rho] : 0.0f;
     if (a.forces) {  // the forces' results as stored (rounded to T)
-      if (row_in && col0) {
+      if (in0) {
         force_uv(sp, gi, gj, t.u0[rho], t.v0[rho]);
         t.u0[rho] = rnd<T>(t.u0[rho]);
         t.v0[rho] = rnd<T>(t.v0[rho]);
       }
-      if (row_in && col1) {
+      if (in1) {
         force_uv(sp, gi + 1, gj, t.u1[rho], t.v1[rho]);
         t.u1[rho] = rnd<T>(t.u1[rho]);
         t.v1[rho] = rnd<T>(t.v1[rho]);
       }
     }
-    if constexpr (PRESSURE) {
-      t.s0[rho] = (row_in && col0 && !a.forces) ? p[gk] : 0.0f;
-      t.s1[rho] = (row_in && col1 && !a.forces) ? p[gk + 1] : 0.0f;
+    if constexpr (PRESSURE) {  // the forces zero the pressure
+      t.s0[rho] = (in0 && !a.forces) ? t.s0[rho] : 0.0f;
+      t.s1[rho] = (in1 && !a.forces) ? t.s1[rho] : 0.0f;
     } else {
       t.s0[rho] = t.s1[rho] = 0.0f;
     }
     // the gate bytes of cells 2l and 2l+1 (static, built with the flags)
-    if (row_in && col0 && col1) {
-      t.gw[rho] = *reinterpret_cast<const uint16_t*>(gates + gk);
-    } else {
-      t.gw[rho] = ((row_in && col0) ? (uint32_t)gates[gk] : 0u) |
-                  ((row_in && col1) ? (uint32_t)gates[gk + 1] << 8 : 0u);
-    }
+    t.gw[rho] = (in0 ? (graw[rho] & 0xFFu) : 0u) | (in1 ? (graw[rho] & 0xFF00u) : 0u);
     t.slow |= (__any((t.gw[rho] & G_SLOW) != 0) ? 1u : 0u) << (2 * rho);
     t.slow |= (__any((t.gw[rho] & (G_SLOW << 8)) != 0) ? 1u : 0u) << (2 * rho + 1);
   }
