@@ -34,7 +34,7 @@ HBM_PEAK_GBS = 8000.0
 # per (kernel, tile width, tile height, sweeps per launch, W, H); summary and
 # commands in profiles/step_1920x1080_n50_fp32.md.
 MEASURED_LAUNCH_TRAFFIC = {
-    (2, 108, 44, 5, 1920, 1080): 38_126_501,
+    (2, 108, 44, 5, 1920, 1080): 38_077_860,
 }
 METRIC = "Gcell-steps/s at 1920x1080 n=50; % HBM roofline; p50/p99 batched step latency"
 
