@@ -107,22 +107,32 @@ __device__ __attribute__((always_inline)) inline void forces_tile(const StepPara
   asm volatile("" : "+v"(tid));
   const int i = bx * kAdvTX + (tid & 63);
   if (i >= sp.W) return;
+  // every load first (rows clamped into the grid), so a wavefront waits for
+  // HBM once rather than once per row
+  constexpr int M = TY / NY;
+  float uu[M], vv[M], s0[M];
 #pragma unroll
-  for (int m = 0; m < TY / NY; m++) {
+  for (int m = 0; m < M; m++) {
+    const long k = (long)min(by * TY + (tid >> 6) + NY * m, sp.H - 1) * sp.P + i;
+    uu[m] = do_uv ? ldf(u, k) : 0.0f;
+    vv[m] = do_uv ? ldf(v, k) : 0.0f;
+    s0[m] = do_smoke ? ldf(smoke, k) : 0.0f;
+  }
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int m = 0; m < M; m++) {
     const int j = by * TY + (tid >> 6) + NY * m;
     if (j >= sp.H) continue;
     const long k = (long)j * sp.P + i;
     if (do_uv) {
-      float uu = ldf(u, k), vv = ldf(v, k);
-      force_uv(sp, i, j, uu, vv);
-      stf(u, k, uu);
-      stf(v, k, vv);
+      force_uv(sp, i, j, uu[m], vv[m]);
+      stf(u, k, uu[m]);
+      stf(v, k, vv[m]);
       if (sp.pressure_on) p[k] = 0.0f;
     }
     if (do_smoke) {
-      const float s0 = ldf(smoke, k);
-      const float s1 = force_smoke(sp, i, j, s0);
-      if (__float_as_uint(s1) != __float_as_uint(s0)) stf(smoke, k, s1);
+      const float s1 = force_smoke(sp, i, j, s0[m]);
+      if (__float_as_uint(s1) != __float_as_uint(s0[m])) stf(smoke, k, s1);
     }
   }
 }

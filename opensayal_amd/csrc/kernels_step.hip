@@ -90,11 +90,13 @@ __global__ void k_build_gates(MaskParams m, const uint8_t* flags, uint8_t* gates
   gates[k] = (uint8_t)gate_byte(f, fr, ft);
 }
 
-// ---- P1: forces over 64 x 4 tiles (forces_tile) -------------------------
+// ---- P1: forces over 64 x 16 tiles (forces_tile), four rows per thread --
+constexpr int kForcesTY = 16;
 template <typename T>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(kBX * kBY)
     k_forces(StepParams sp, int do_uv, int do_smoke, T* u, T* v, T* smoke, float* p) {
-  forces_tile<T, kBY, kBY>(sp, do_uv, do_smoke, u, v, smoke, p, blockIdx.x, blockIdx.y);
+  static_assert(kBX == kAdvTX && kForcesTY % kBY == 0, "forces tile shape");
+  forces_tile<T, kForcesTY, kBY>(sp, do_uv, do_smoke, u, v, smoke, p, blockIdx.x, blockIdx.y);
 }
 
 // ---- P5: thrust::reduce min/max of pressure (src/fluid.cu:778-787) -------
@@ -403,7 +405,9 @@ void step_t(const StepParams& sp_in, const FieldPtrs& f, int* uv, int* sm,
   const bool fuse_smoke = sp.smoke_on;
   sp.forces_in_projection = fuse_uv ? 1 : 0;
   if (!fuse_uv || !fuse_smoke)
-    hipLaunchKernelGGL(k_forces<T>, cell_grid(sp.W, sp.H), dim3(kBX * kBY), 0, s, sp,
+    hipLaunchKernelGGL(k_forces<T>,
+                       dim3((sp.W + kBX - 1) / kBX, (sp.H + kForcesTY - 1) / kForcesTY),
+                       dim3(kBX * kBY), 0, s, sp,
                        fuse_uv ? 0 : 1, fuse_smoke ? 0 : 1, (T*)f.u[*uv], (T*)f.v[*uv],
                        (T*)f.smoke[*sm], f.p);
   int cur = *uv;
