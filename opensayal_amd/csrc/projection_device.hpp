@@ -249,8 +249,8 @@ __device__ __attribute__((always_inline)) inline void reg_rows_from(RegTile<R>& 
 //    the next barrier.  The caller publishes and reads the rows as loaded
 //    before the first phase.
 // The two give the same bits.  The persistent batch kernel measured faster
-// with one barrier; the per-grid kernel did not, and with one its 4-row form
-// needs more than 80 VGPRs (one wavefront fewer per SIMD).
+// with one barrier; the per-grid kernel measured no faster at 8 rows and
+// slower at 4 (its extra live registers cost a wavefront per SIMD).
 template <typename T, int R, int PB, bool PRESSURE, bool ONE_SYNC>
 __device__ __attribute__((always_inline)) inline void reg_phase(RegTile<R>& t, RegShared& sh,
                                                                 float& vt0, float& vt1, int lane,
