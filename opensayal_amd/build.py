@@ -5,7 +5,8 @@ multiply-add, so float arithmetic rounds after each operation as the
 reference's source states) and without fast-math (division and square root
 stay correctly rounded).  ``-packed-fp32-ops`` is switched off for device code.
 
-Usage: ``python -m opensayal_amd.build`` (also called by ``__graft_entry__.build``).
+Usage: ``python -m opensayal_amd.build [--incremental]`` (the full build is
+also called by ``__graft_entry__.build``).
 """
 from __future__ import annotations
 
@@ -49,9 +50,24 @@ DEVICE = [
 ]
 
 
-def _compile(src: str) -> Path:
+def _headers_mtime() -> float:
+    hs = list(CSRC.glob("*.hpp")) + list((ROOT / "include").glob("*.h"))
+    return max(h.stat().st_mtime for h in hs)
+
+
+def _up_to_date(src: Path, out: Path) -> bool:
+    """The object exists and is newer than its source and every header."""
+    if not out.exists():
+        return False
+    t = out.stat().st_mtime
+    return t > src.stat().st_mtime and t > _headers_mtime()
+
+
+def _compile(src: str, incremental: bool = False) -> Path:
     out = OBJ / (Path(src).stem + ".o")
     path = CSRC / src
+    if incremental and _up_to_date(path, out):
+        return out
     if src.endswith(".hip"):
         cmd = [HIPCC, *COMMON, *DEVICE, "-x", "hip", "-c", str(path), "-o", str(out)]
     else:
@@ -62,11 +78,13 @@ def _compile(src: str) -> Path:
     return out
 
 
-def build(verbose: bool = True) -> Path:
-    """Compiles every source and links the shared library; returns its path."""
+def build(verbose: bool = True, incremental: bool = False) -> Path:
+    """Compiles every source (with ``incremental``, only those whose object
+    is older than the source or a header) and links the shared library;
+    returns its path."""
     OBJ.mkdir(exist_ok=True)
     with concurrent.futures.ThreadPoolExecutor(max_workers=min(8, len(SOURCES))) as ex:
-        objs = list(ex.map(_compile, SOURCES))
+        objs = list(ex.map(lambda s: _compile(s, incremental), SOURCES))
     tmp = LIB.with_suffix(".so.tmp")
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp),
            *[str(o) for o in objs]]
@@ -105,4 +123,4 @@ def clean() -> None:
 
 
 if __name__ == "__main__":
-    build()
+    build(incremental="--incremental" in sys.argv[1:])
