@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define OSY_ABI_VERSION 4
+#define OSY_ABI_VERSION 5
 
 typedef enum osy_status {
   OSY_OK = 0,
@@ -249,6 +249,19 @@ int osy_fluid_render_rgba(osy_fluid* fluid, uint32_t* dst, size_t bytes);
  * distance >= 1, 1 <= length <= 65536.  Synchronous. */
 int osy_fluid_traces(osy_fluid* fluid, int32_t distance, int32_t length, float d_t,
                      int32_t* traces_x, int32_t* traces_y, size_t bytes);
+
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (serving loops and harnesses that forbid hipMalloc after set-up):
+ * osy_fluid_traces with the same arguments, checks and results, except that
+ * the device staging of the two arrays is a fixed library-owned device
+ * array (2 MiB per GPU, held until the process ends) instead of a buffer
+ * osy_fluid_traces allocates and grows per handle.  Each of traces_x and
+ * traces_y may hold at most 262144 entries ((W/distance)*(H/distance)*
+ * length); larger calls return OSY_E_INVALID.  All calls on one GPU share
+ * the staging array: calls that overlap on one GPU (from several threads)
+ * race.  Synchronous. */
+int osy_fluid_traces_noalloc(osy_fluid* fluid, int32_t distance, int32_t length, float d_t,
+                             int32_t* traces_x, int32_t* traces_y, size_t bytes);
 
 /* Select the projection launch shape (OSY_E_INVALID if the shape does not
  * fit the kernel's limits).  Captured step graphs are rebuilt on next use. */

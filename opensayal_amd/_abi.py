@@ -124,6 +124,9 @@ PROTOTYPES = {
     "osy_fluid_traces": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32,
                                         ctypes.c_float, ctypes.c_void_p, ctypes.c_void_p,
                                         ctypes.c_size_t]),
+    "osy_fluid_traces_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32,
+                                                ctypes.c_int32, ctypes.c_float, ctypes.c_void_p,
+                                                ctypes.c_void_p, ctypes.c_size_t]),
     "osy_fluid_set_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(OsyTuning)]),
     "osy_fluid_get_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(OsyTuning)]),
     "osy_fluid_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(OsyTuning),

@@ -34,6 +34,7 @@ SOURCES = [
     "kernels_project.hip",
     "kernels_batch.hip",
     "kernels_view.hip",
+    "noalloc.hip",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
