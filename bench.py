@@ -325,7 +325,12 @@ def main() -> int:
                     help="skip the leg with the default viscosity")
     ap.add_argument("--view-reps", type=int, default=20)
     ap.add_argument("--requests", type=int, default=64, help="serving: number of requests")
-    ap.add_argument("--rate", type=float, default=8.0, help="serving: arrivals per second")
+    ap.add_argument("--serving-requests", type=int, default=32,
+                    help="requests of the serving leg of the default run")
+    ap.add_argument("--serving-seed", type=int, default=5,
+                    help="seed of the serving leg's arrivals and requests")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving leg")
+    ap.add_argument("--rate", type=float, default=16.0, help="serving: arrivals per second")
     ap.add_argument("--budget-gb", type=float, default=200.0, help="serving: HBM state budget")
     ap.add_argument("--chunk", type=int, default=16, help="serving: steps between admissions")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -476,6 +481,9 @@ def main() -> int:
         # small ones (64..384 wide), each as one batch against the per-grid path
         out["batched"] = batch_leg(wl.batched_specs(), args.batch_steps, 3)
         out["batched_mixed_small"] = batch_leg(wl.small_specs(), args.batch_steps * 5, 5)
+    if not args.no_serving and args.workload == "full":
+        out["serving"] = serving_leg(args.serving_requests, args.rate, args.budget_gb,
+                                     args.chunk, args.serving_seed)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(cfg, args.seed)
     else:
@@ -484,25 +492,44 @@ def main() -> int:
     return 0
 
 
-def serving_main(args) -> int:
-    """Poisson arrivals of mixed requests served by continuous batching."""
+def serving_leg(requests: int, rate_hz: float, budget_gb: float, chunk: int,
+                seed: int) -> dict:
+    """Poisson arrivals of requests from the serving workload's distribution
+    (width log-uniform 64..2048 at 16:9, 50..2000 steps, n from {20, 50,
+    100}, FP32 or FP16 storage), served by continuous batching under an HBM
+    budget: the resident requests step together, one persistent launch per
+    batch step, each batch step timed by an event pair on the batch's
+    stream; requests arriving meanwhile are created and uploaded while a
+    chunk runs and join at its end.  Throughput is all requests' cell-steps
+    over the wall time from the first arrival to the last completion."""
     from opensayal_amd import workloads as wl
     from opensayal_amd.serving import DeviceBackend, Server
 
-    arrivals = wl.serving_specs(count=args.requests, seed=args.seed, rate_hz=args.rate)
-    backend = DeviceBackend()
+    arrivals = wl.serving_specs(count=requests, seed=seed, rate_hz=rate_hz)
+    backend = DeviceBackend(max_chunk=chunk)
     try:
-        rep = Server(backend, budget_bytes=int(args.budget_gb * 1e9),
-                     max_chunk=args.chunk).run(arrivals)
+        rep = Server(backend, budget_bytes=int(budget_gb * 1e9), max_chunk=chunk).run(arrivals)
     finally:
         backend.close()
     s = rep.summary()
-    out = {"metric": METRIC, "value": round(s["gcell_steps_s_wall"], 4), "unit": "Gcell-steps/s",
+    out = {k: (round(v, 5) if isinstance(v, float) else v) for k, v in s.items()}
+    out.update({"value": out["gcell_steps_s_wall"], "unit": "Gcell-steps/s",
+                "gpu_busy_frac": round(s["gpu_busy_s"] / max(s["wall_s"], 1e-9), 4),
+                "rate_hz": rate_hz, "budget_gb": budget_gb, "max_chunk": chunk, "seed": seed,
+                "widths": [min(r.spec.cfg.width for r in rep.requests),
+                           max(r.spec.cfg.width for r in rep.requests)]})
+    return out
+
+
+def serving_main(args) -> int:
+    """Poisson arrivals of mixed requests served by continuous batching."""
+    s = serving_leg(args.requests, args.rate, args.budget_gb, args.chunk, args.seed)
+    out = {"metric": METRIC, "value": s["value"], "unit": "Gcell-steps/s",
            "n_gpus": 1, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
            "dtype": "f32/f16 storage", "data": "synthetic requests: Poisson arrivals",
            "config": {"workload": "serving", "requests": args.requests, "rate_hz": args.rate,
                       "budget_gb": args.budget_gb, "max_chunk": args.chunk, "seed": args.seed},
-           "serving": {k: (round(v, 5) if isinstance(v, float) else v) for k, v in s.items()}}
+           "serving": s}
     print(json.dumps(out), flush=True)
     return 0
 

@@ -8,9 +8,12 @@ simulation's state resident in HBM as a member of one ``Batch``, and steps
 all members together in chunks: one persistent kernel launch per step (per
 storage group), so the work of every resident simulation overlaps on the
 device.  A chunk never runs past the step count of any member, so each
-request runs exactly its steps; finished requests are collected (fields
-copied to the host if asked), leave the batch and free their memory, and
-requests that arrived meanwhile are admitted before the next chunk.
+request runs exactly its steps.  While a chunk runs on the GPU, the host
+admits the requests that arrived meanwhile: it creates their handles and
+uploads their initial state on their own streams, and they join the batch
+at the end of the chunk, when finished requests are collected (fields
+copied to the host if asked), leave the batch and free their memory.  Each
+batch step is timed by an event pair on the batch's stream.
 
 Each request's result is bit-identical to stepping it alone, because a batch
 member's fields are (see ``Batch``).
@@ -89,9 +92,16 @@ class ServingReport:
 
 
 class DeviceBackend:
-    """The GPU side: one ``Batch`` whose members are the resident requests."""
+    """The GPU side: one ``Batch`` whose members are the resident requests.
 
-    def __init__(self, tuning: Optional[dict] = None, collect: Sequence[str] = ()):
+    ``prepare`` creates a request's handle and uploads its state (on the
+    handle's own stream, so it may run while the batch steps); ``join`` adds
+    it to the batch between chunks; ``launch`` enqueues steps with an event
+    pair around each and returns at once; ``wait`` returns each step's GPU
+    milliseconds."""
+
+    def __init__(self, tuning: Optional[dict] = None, collect: Sequence[str] = (),
+                 max_chunk: int = 64):
         from . import Batch, Event, Stream
 
         self.batch = Batch()
@@ -99,28 +109,38 @@ class DeviceBackend:
             self.batch.set_tuning(**tuning)
         self.collect = tuple(collect)
         self.stream = Stream()
-        self._events = (Event(), Event())
+        self._events = [Event() for _ in range(max_chunk + 1)]
+        self._launched = 0
 
     def bytes(self, spec: SimSpec) -> int:
         from . import device_bytes
 
         return device_bytes(spec.cfg)
 
-    def admit(self, spec: SimSpec):
+    def prepare(self, spec: SimSpec):
         from .workloads import start
 
-        fl = start(spec)
-        self.batch.add(fl)
-        return fl
+        return start(spec)
 
-    def step(self, steps: int) -> float:
-        """Runs `steps` batch steps; returns the GPU milliseconds they took."""
-        a, b = self._events
-        a.record(self.stream)
-        self.batch.step_async(steps, stream=self.stream)
-        b.record(self.stream)
+    def join(self, handle) -> None:
+        self.batch.add(handle)
+
+    def launch(self, steps: int) -> None:
+        if steps > len(self._events) - 1:
+            from . import Event
+
+            self._events += [Event() for _ in range(steps + 1 - len(self._events))]
+        ev = self._events
+        ev[0].record(self.stream)
+        for k in range(steps):
+            self.batch.step_async(1, stream=self.stream)
+            ev[k + 1].record(self.stream)
+        self._launched = steps
+
+    def wait(self) -> List[float]:
         self.batch.synchronize()
-        return a.elapsed_ms(b)
+        ev = self._events
+        return [ev[k].elapsed_ms(ev[k + 1]) for k in range(self._launched)]
 
     def retire(self, handle) -> Optional[Dict[str, np.ndarray]]:
         out = {n: handle.get_field(n) for n in self.collect} if self.collect else None
@@ -157,30 +177,43 @@ class Server:
         waiting: List[Request] = []
         active: List[Request] = []
         rep = ServingReport(requests=reqs)
-        used = 0
+        used = 0  # device bytes of the resident and prepared requests
         t0 = clock()
         virtual = 0.0  # the clock when not running in real time: GPU time only
 
         def now() -> float:
             return clock() - t0 if realtime else virtual
 
-        while pending or waiting or active:
+        ready: List[Request] = []  # prepared while a chunk ran; join at its end
+
+        def admit() -> None:
+            nonlocal used
+            for r in list(waiting):
+                if len(active) + len(ready) >= self.max_slots:
+                    break
+                if used + r.bytes <= self.budget:
+                    r.handle = self.backend.prepare(r.spec)
+                    r.admitted = now()
+                    used += r.bytes
+                    ready.append(r)
+                    waiting.remove(r)
+            rep.peak_bytes = max(rep.peak_bytes, used)
+            rep.peak_members = max(rep.peak_members, len(active) + len(ready))
+
+        while pending or waiting or ready or active:
             t = now()
             while pending and pending[0].arrival <= t:
                 waiting.append(pending.popleft())
-            # admission: arrival order, skipping requests that do not fit yet
-            for r in list(waiting):
-                if len(active) >= self.max_slots:
-                    break
-                if used + r.bytes <= self.budget:
-                    r.handle = self.backend.admit(r.spec)
-                    r.admitted = now()
-                    used += r.bytes
-                    active.append(r)
-                    waiting.remove(r)
-            rep.peak_bytes = max(rep.peak_bytes, used)
-            rep.peak_members = max(rep.peak_members, len(active))
+            for r in ready:
+                self.backend.join(r.handle)
+                active.append(r)
+            ready.clear()
             if not active:
+                # nothing on the GPU: admit now (arrival order, skipping
+                # requests that do not fit yet) and start them
+                admit()
+                if ready:
+                    continue
                 if pending:
                     if realtime:
                         sleep(max(0.0, pending[0].arrival - now()))
@@ -189,12 +222,18 @@ class Server:
                     continue
                 raise RuntimeError("requests wait but none can be admitted")
             chunk = min(self.max_chunk, min(r.remaining for r in active))
-            ms = self.backend.step(chunk)
+            self.backend.launch(chunk)
+            # while the chunk runs: requests that arrived are admitted
+            t = now()
+            while pending and pending[0].arrival <= t:
+                waiting.append(pending.popleft())
+            admit()
+            step_ms = self.backend.wait()
             rep.chunks += 1
-            rep.gpu_s += ms / 1e3
-            rep.step_ms.extend([ms / chunk] * chunk)
+            rep.gpu_s += sum(step_ms) / 1e3
+            rep.step_ms.extend(step_ms)
             if not realtime:
-                virtual += ms / 1e3
+                virtual += sum(step_ms) / 1e3
             for r in active:
                 r.steps_done += chunk
             for r in [r for r in active if r.remaining == 0]:

@@ -213,3 +213,39 @@ def test_destroyed_member_leaves_batch():
     ref = started(make_config(80, 45, projection_n=9), 1)
     ref.step(2)
     assert_same(state(keep), state(ref), "after a member was destroyed")
+
+
+def test_batched_workload_256_grids():
+    """The batched workload itself: 256 grids 128..1024 wide (height 9W/16),
+    n from {20, 50, 100}, obstacle and inflow drawn per grid, stepped by one
+    persistent launch per step.  After 3 batch steps every member equals the
+    same grid stepped alone; the smallest grid, the largest and the smallest
+    n = 100 grid equal the oracle."""
+    specs = wl.batched_specs()
+    assert len(specs) == 256
+    batched = [wl.start(s) for s in specs]
+    alone = [wl.start(s) for s in specs]
+    with Batch() as b:
+        for fl in batched:
+            b.add(fl)
+        b.step(3)
+        assert b.info["count"] == 256
+        for fl in alone:
+            fl.step(3)
+        for i, (x, y) in enumerate(zip(batched, alone)):
+            assert_same(state(x), state(y), f"grid {i} ({specs[i].cfg.width}x{specs[i].cfg.height})")
+    cells = lambda k: specs[k].cfg.width * specs[k].cfg.height
+    hundred = [k for k in range(256) if specs[k].cfg.projection_n == 100]
+    picks = {min(range(256), key=cells), max(range(256), key=cells), min(hundred, key=cells)}
+    for k in sorted(picks):
+        cfg = specs[k].cfg
+        o = OracleFluid(cfg, nthreads=16)
+        u, v, s = wl.initial_state(cfg, specs[k].seed)
+        o.set_state(u=u, v=v, smoke=s)
+        o.update(steps=3)
+        for name, attr in (("vel_x", "u"), ("vel_y", "v"), ("smoke", "smoke")):
+            got = batched[k].get_field(name)
+            assert np.array_equal(got.view(np.uint32), getattr(o, attr).view(np.uint32)), \
+                (k, cfg.width, cfg.height, cfg.projection_n, name)
+    for fl in batched + alone:
+        fl.close()

@@ -1,7 +1,9 @@
-"""The headless fluid-simulator program on the GPU: its .npy field dumps,
-frames and path lines equal what the Python API computes for the same
-config and steps (both call Fluid::update once per frame through the same C
-ABI), and its performance record is valid JSON."""
+"""The headless fluid-simulator program on the GPU (the reference's main
+loop without a window, src/main.cu:42-107): its .npy field dumps, frames
+and path lines equal the oracle's for the same configuration and steps
+(the oracle's configuration is written out field by field from the JSON,
+not read by the product's parser), and what the Python API computes; its
+performance record is valid JSON."""
 import json
 import subprocess
 
@@ -10,6 +12,8 @@ import pytest
 
 from opensayal_amd import ConfigParser, Fluid
 from opensayal_amd._abi import CLI_PATH
+from oracle.oracle import OracleFluid
+from tests.helpers import make_config
 
 pytestmark = pytest.mark.gpu
 
@@ -78,6 +82,54 @@ def test_cli_dumps_equal_python_api(tmp_path, storage):
         rgb = np.frombuffer(ppm[len(head):], np.uint8).reshape(74, 130, 3)
         assert np.array_equal(rgb[..., 0], (frame >> 24).astype(np.uint8))
         assert np.array_equal(rgb[..., 2], ((frame >> 8) & 255).astype(np.uint8))
+
+
+def oracle_config():
+    """CONFIG as the oracle's configuration, key by key (the defaults of
+    src/config_parser.cpp for the keys CONFIG leaves out: smoke on, real
+    time off, cell_pixel_size 1, block sizes 64 x 1)."""
+    sim = CONFIG["sim"]
+    wt = sim["wind_tunnel"]
+    ob = sim["obstacle"]
+    return make_config(
+        sim["width"], sim["height"], cell_size=float(sim["cell_size"]),
+        enable_pressure=int(sim["enable_pressure"]), enable_smoke=int(sim["enable_smoke"]),
+        enable_drain=int(sim["enable_drain"]), projection_n=sim["projection"]["n"],
+        projection_o=sim["projection"]["o"], wind_speed=wt["speed"],
+        pipe_height=wt["pipe_height"], smoke_height=wt["smoke_height"],
+        smoke_count=wt["smoke_count"], smoke_length=wt["smoke_length"],
+        wind_smoke=wt["smoke"], g=sim["physics"]["g"], d_t=sim["time"]["d_t"],
+        enable_decay=int(sim["smoke"]["enable_decay"]), decay_rate=sim["smoke"]["decay_rate"],
+        obstacle_enable=int(ob["enable"]), obstacle_center_x=ob["center_x"],
+        obstacle_center_y=ob["center_y"], obstacle_radius=ob["radius"],
+        density=CONFIG["fluid"]["density"], drag_coeff=CONFIG["fluid"]["drag_coeff"],
+        viscosity=float(CONFIG["fluid"]["viscosity"]))
+
+
+def test_cli_dumps_equal_oracle(tmp_path):
+    """fluid-simulator for 9 steps from the zero state (Fluid::Fluid,
+    src/fluid.cu:99-110): the dumped u, v, smoke, pressure, frame and path
+    lines after steps 4 and 9 equal the oracle's, bit for bit."""
+    path = tmp_path / "OpenSayal.conf.json"
+    path.write_text(json.dumps(CONFIG))
+    out = tmp_path / "dump"
+    run_cli("--config", str(path), "--steps", "9", "--dump", str(out), "--dump-every", "4",
+            "--frame")
+    cfg = oracle_config()
+    o = OracleFluid(cfg, nthreads=8)
+    for step in range(1, 10):
+        o.update()
+        if step not in (4, 8, 9):
+            continue
+        for name, attr in (("vel_x", "u"), ("vel_y", "v"), ("smoke", "smoke"),
+                           ("pressure", "p")):
+            got = load(out, step, name)
+            assert np.array_equal(got.view(np.uint32), getattr(o, attr).view(np.uint32)), \
+                (step, name)
+        assert np.array_equal(load(out, step, "frame"), o.render_rgba()), step
+        tx, ty = o.traces(9, 12)
+        assert np.array_equal(load(out, step, "traces_x"), tx), step
+        assert np.array_equal(load(out, step, "traces_y"), ty), step
 
 
 def test_cli_compare_performance(tmp_path):

@@ -10,29 +10,42 @@ from tests.helpers import make_config
 
 
 class FakeBackend:
-    """Counts steps per resident handle; 1 ms of 'GPU' per step per member."""
+    """Counts steps per resident handle; 1 ms of 'GPU' per step per member.
+    Prepared handles hold memory but step only once they join."""
 
     def __init__(self):
+        self.prepared = {}
         self.resident = {}
         self.log = []
         self.next = 0
+        self.pending_steps = 0
 
     def bytes(self, spec):
         return spec.cfg.width * spec.cfg.height * 10
 
-    def admit(self, spec):
+    def held(self):
+        return list(self.prepared.values()) + [s for s, _ in self.resident.values()]
+
+    def prepare(self, spec):
         h = self.next
         self.next += 1
-        self.resident[h] = [spec, 0]
-        self.log.append(("admit", h, sum(self.bytes(s) for s, _ in self.resident.values()),
-                         len(self.resident), spec.cfg.width))
+        self.prepared[h] = spec
+        held = self.held()
+        self.log.append(("admit", h, sum(self.bytes(s) for s in held), len(held),
+                         spec.cfg.width))
         return h
 
-    def step(self, steps):
+    def join(self, h):
+        self.resident[h] = [self.prepared.pop(h), 0]
+
+    def launch(self, steps):
         for v in self.resident.values():
             v[1] += steps
         self.log.append(("step", steps, [v[1] for v in self.resident.values()]))
-        return float(steps * len(self.resident))
+        self.pending_steps = steps
+
+    def wait(self):
+        return [float(len(self.resident))] * self.pending_steps
 
     def retire(self, h):
         spec, done = self.resident.pop(h)
@@ -105,3 +118,17 @@ def test_summary_fields():
     assert s["completed"] == 5 and s["cell_steps"] == 5 * 16 * 9 * 10
     assert s["step_ms_p99"] >= s["step_ms_p50"] > 0
     assert s["completion_s_p99"] >= s["completion_s_p50"] > 0
+
+
+def test_admission_overlaps_the_running_chunk():
+    """A request that arrives while others run is prepared during a chunk
+    (after that chunk's launch, before its wait) and steps from the next."""
+    arrivals = [(0.0, spec(30, 20, 40)), (0.0025, spec(20, 12, 8))]
+    be, rep = run(arrivals, max_chunk=4)
+    order = [e[0] for e in be.log]
+    i_admit_b = [k for k, e in enumerate(be.log) if e[0] == "admit" and e[4] == 20][0]
+    assert order[i_admit_b - 1] == "step"        # prepared while a chunk ran
+    launched_before = [e for e in be.log[:i_admit_b] if e[0] == "step"]
+    assert all(len(e[2]) == 1 for e in launched_before)  # B not stepped before it joined
+    b = rep.requests[1]
+    assert b.result == {"steps": 8} and b.admitted >= b.arrival
