@@ -39,6 +39,13 @@ MEASURED_LAUNCH_TRAFFIC = {
 METRIC = "Gcell-steps/s at 1920x1080 n=50; % HBM roofline; p50/p99 batched step latency"
 
 
+def launch_traffic(tuning: dict, W: int, H: int):
+    """Measured HBM bytes of one projection launch of this shape, or None."""
+    return MEASURED_LAUNCH_TRAFFIC.get((tuning["projection_kernel"], tuning["tile_width"],
+                                        tuning["tile_height"], tuning["sweeps_per_launch"],
+                                        W, H))
+
+
 def step_bytes_per_cell(n: int) -> int:
     """Algorithmic bytes per cell-step, FP32, pressure off: forces 8 (v read
     and written), 17 per sweep, u/v advection 17, smoke advection 17."""
@@ -118,7 +125,44 @@ def pct(xs, q):
     return float(np.percentile(np.asarray(xs, dtype=np.float64), q))
 
 
-def batch_leg(specs, steps: int, warmup: int, plain_steps: int = 3, autotune: bool = True) -> dict:
+KIND_NAMES = ("forces", "projection", "velocity_advection", "smoke_advection")
+
+
+def item_timing(b) -> dict:
+    """One batch step with per-item timing: per stage kind, the time items
+    waited for their dependencies after being claimed and the time their
+    body ran (microseconds), and the step's span from the first claim to
+    the last completion."""
+    import numpy as np
+
+    b.set_timing(True)
+    b.step(2)  # the first rebuilds the tables with the stamps
+    r = b.item_times()
+    b.set_timing(False)
+    wait = (r["ready"] - r["claim"]) / 1e3
+    body = (r["done"] - r["ready"]) / 1e3
+    span = float(r["done"].max() - r["claim"].min()) / 1e3
+    out = {"span_us": round(span, 2), "items": int(len(wait)),
+           "cus": int(len(np.unique(r["hw_id"]))), "tick_ns": r["tick_ns"], "kinds": {}}
+    for k, name in enumerate(KIND_NAMES):
+        m = r["kind"] == k
+        if not m.any():
+            continue
+        out["kinds"][name] = {
+            "items": int(m.sum()),
+            "wait_us_mean": round(float(wait[m].mean()), 3),
+            "wait_us_p99": round(pct(list(wait[m]), 99), 3),
+            "body_us_mean": round(float(body[m].mean()), 3),
+            "body_us_p50": round(pct(list(body[m]), 50), 3),
+            "body_us_p99": round(pct(list(body[m]), 99), 3),
+            "wait_us_sum": round(float(wait[m].sum()), 1),
+            "body_us_sum": round(float(body[m].sum()), 1),
+        }
+    return out
+
+
+def batch_leg(specs, steps: int, warmup: int, plain_steps: int = 3, autotune: bool = True,
+              timing: bool = False) -> dict:
     """Steps every simulation of `specs` together as one Batch (one persistent
     launch per step, replayed from a HIP graph), timing each step with HIP
     events on the stream the steps run on; then the same simulations one
@@ -146,6 +190,7 @@ def batch_leg(specs, steps: int, warmup: int, plain_steps: int = 3, autotune: bo
     per = [evs[k].elapsed_ms(evs[k + 1]) for k in range(steps)]
     total_ms = evs[0].elapsed_ms(evs[-1])
     info = b.info
+    timed = item_timing(b) if timing else None
     b.close()
 
     # the per-grid path: each simulation alone, sequentially on one stream
@@ -181,6 +226,7 @@ def batch_leg(specs, steps: int, warmup: int, plain_steps: int = 3, autotune: bo
         "step_roofline": {"alg_bytes_per_step": int(alg), "achieved_GBs": round(achieved, 1),
                           "frac": round(achieved / HBM_PEAK_GBS, 4)},
         "setup_s": round(setup_s, 2),
+        **({"item_timing": timed} if timed else {}),
     }
 
 
@@ -311,7 +357,11 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=300)
     ap.add_argument("--warmup", type=int, default=30)
     ap.add_argument("--workload", default="full",
-                    choices=["full", "small", "batched", "mixed_small", "serving"])
+                    choices=["full", "small", "batched", "mixed_small", "serving",
+                             "lone_batch"])
+    ap.add_argument("--item-timing", action="store_true",
+                    help="batched workloads: add one step with per-item timing and report "
+                         "wait and body time per stage kind")
     ap.add_argument("--batch-steps", type=int, default=20,
                     help="timed batch steps of the batched legs of the default run")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched legs")
@@ -338,6 +388,9 @@ def main() -> int:
                          "FETCH_SIZE / WRITE_SIZE counters)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default projection shape (for profiling runs)")
+    ap.add_argument("--shape", default=None,
+                    help="projection launch shape KERNEL,TILE_W,TILE_H,SWEEPS instead of "
+                         "the autotuner's (for profiling one shape)")
     ap.add_argument("--sweep", action="store_true",
                     help="time a grid of projection launch shapes instead")
     args = ap.parse_args()
@@ -359,9 +412,11 @@ def main() -> int:
                                              "bytes_written": nbytes, "copies": 5}}), flush=True)
         return 0
 
-    if args.workload in ("batched", "mixed_small"):
-        specs = wl.batched_specs() if args.workload == "batched" else wl.small_specs()
-        out = batch_leg(specs, args.steps, args.warmup, autotune=not args.no_autotune)
+    if args.workload in ("batched", "mixed_small", "lone_batch"):
+        specs = {"batched": wl.batched_specs, "mixed_small": wl.small_specs,
+                 "lone_batch": lambda: [wl.SimSpec(wl.plain_full(), args.seed, 0)]}[args.workload]()
+        out = batch_leg(specs, args.steps, args.warmup, autotune=not args.no_autotune,
+                        timing=args.item_timing)
         out.update({"metric": METRIC, "n_gpus": 1, "higher_is_better": True,
                     "scaling": "weak", "vs_baseline": None, "dtype": "f32",
                     "data": "synthetic: seeded +-0.01 velocity noise, smoke 1 on the inflow",
@@ -386,7 +441,11 @@ def main() -> int:
         return 0
 
     # the library's autotuner picks the projection launch shape (not timed)
-    if args.no_autotune:
+    if args.shape:
+        kern, tw, th, k = (int(x) for x in args.shape.split(","))
+        fl.set_tuning(projection_kernel=kern, tile_width=tw, tile_height=th, sweeps_per_launch=k)
+        tuning, tune_ms = fl.tuning, float("nan")
+    elif args.no_autotune:
         tuning, tune_ms = fl.tuning, float("nan")
     else:
         tuning, tune_ms = fl.autotune()
@@ -427,18 +486,27 @@ def main() -> int:
     per_launch_ms = a.elapsed_ms(b) / launches
     alg_bytes_per_launch = 17 * spl * args.kernel_reps * W * H / launches
     achieved = alg_bytes_per_launch / (per_launch_ms / 1e3) / 1e9
+    if launch_traffic(tuning, W, H) is None:
+        # a shape without a FETCH_SIZE / WRITE_SIZE measurement: say so, and
+        # carry the shapes that were measured rather than a silent null
+        traffic_note = {"traffic_missing": "no rocprofv3 FETCH_SIZE/WRITE_SIZE measurement "
+                                           "for this launch shape",
+                        "traffic_measured": [{"shape": list(k[:4]), "grid": list(k[4:]),
+                                              "bytes": b}
+                                             for k, b in MEASURED_LAUNCH_TRAFFIC.items()]}
+    else:
+        traffic_note = {}
     roofline = {
         "bound": "hbm",
         "achieved": round(achieved, 1),
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": MEASURED_LAUNCH_TRAFFIC.get(
-            (tuning["projection_kernel"], tuning["tile_width"], tuning["tile_height"],
-             tuning["sweeps_per_launch"], W, H)),
+        "traffic": launch_traffic(tuning, W, H),
         "kernel": fl.projection_kernel_name,
         "avg_launch_us": round(per_launch_ms * 1e3, 2),
         "alg_bytes_per_launch": int(alg_bytes_per_launch),
+        **traffic_note,
     }
     step_alg = step_bytes_per_cell(n) * W * H * args.steps / (gpu_ms / 1e3) / 1e9
     out = {

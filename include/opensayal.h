@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define OSY_ABI_VERSION 5
+#define OSY_ABI_VERSION 6
 
 typedef enum osy_status {
   OSY_OK = 0,
@@ -160,6 +160,23 @@ typedef struct osy_batch_info {
 } osy_batch_info;
 
 typedef struct osy_batch osy_batch;
+
+/* One work item of a batch's last step, as osy_batch_item_times reports it:
+ * the item's member (index in the order handles were added), stage kind
+ * (0 forces, 1 projection launch, 2 velocity advection, 3 smoke advection),
+ * stage index within the member's step and tile; the wall-clock ticks at
+ * which a workgroup claimed it, found its dependencies complete and finished
+ * it; and that workgroup's hardware id (__smid(): XCD, shader engine, CU). */
+typedef struct osy_batch_item_time {
+  int32_t member;
+  int32_t kind;
+  int32_t stage;
+  int32_t tile;
+  uint64_t claim;
+  uint64_t ready;
+  uint64_t done;
+  uint64_t hw_id;
+} osy_batch_item_time;
 
 /* ---- configuration: replaces ConfigParser (src/config_parser.cpp:11-184) */
 
@@ -317,6 +334,18 @@ int osy_batch_get_tuning(const osy_batch* batch, osy_batch_tuning* tuning);
  * it to `chosen` and its time per batch step to `ms` (either may be NULL). */
 int osy_batch_autotune(osy_batch* batch, osy_batch_tuning* chosen, double* ms);
 int osy_batch_get_info(const osy_batch* batch, osy_batch_info* info);
+/* Per-item timing of the batch's persistent launches (no counterpart in the
+ * reference; a profiling aid).  Enabled, every launch stamps each of its
+ * work items (four stores per item); the results are unchanged. */
+int osy_batch_set_timing(osy_batch* batch, int32_t enable);
+/* Waits for the batch, then writes the records of its last step's items
+ * (every storage group's launch, in group order) to `out` (room for
+ * `capacity`), their number to `count` and the wall-clock tick in
+ * nanoseconds to `tick_ns`.  OSY_E_STATE when timing is off or no step ran
+ * since it was enabled; OSY_E_INVALID when `capacity` is too small (`count`
+ * then holds the number needed). */
+int osy_batch_item_times(osy_batch* batch, osy_batch_item_time* out, int64_t capacity,
+                         int64_t* count, double* tick_ns);
 
 /* ---- streams and events of the library's HIP runtime (no counterpart in
  * the reference).  For callers that order or time the solver's work without

@@ -21,8 +21,8 @@ from typing import Optional
 
 import numpy as np
 
-from ._abi import (FIELDS, STORAGE, OpenSayalError, OsyBatchInfo, OsyBatchTuning, OsyConfig,
-                   OsyFluidInfo, OsySource, OsyTuning, check, lib)
+from ._abi import (FIELDS, STORAGE, OpenSayalError, OsyBatchInfo, OsyBatchItemTime,
+                   OsyBatchTuning, OsyConfig, OsyFluidInfo, OsySource, OsyTuning, check, lib)
 
 __all__ = ["Batch", "Config", "ConfigParser", "Event", "Fluid", "Mouse", "Source", "Stream",
            "OpenSayalError", "default_config", "device_bytes", "FIELDS", "STORAGE"]
@@ -411,6 +411,30 @@ class Batch:
         check(lib().osy_batch_autotune(self._h, ctypes.byref(t), ctypes.byref(ms)))
         return {"rows": t.rows, "sweeps_per_tile": t.sweeps_per_tile,
                 "occupancy": t.occupancy}, ms.value
+
+    def set_timing(self, enable: bool = True) -> None:
+        """Stamp every work item of every launch (claim, dependencies
+        complete, done; results unchanged)."""
+        check(lib().osy_batch_set_timing(self._h, 1 if enable else 0))
+
+    def item_times(self) -> dict:
+        """The last step's work items as arrays: member, kind (0 forces, 1
+        projection, 2 velocity advection, 3 smoke advection), stage, tile,
+        hw_id, and claim / ready / done in nanoseconds from the step's first
+        claim."""
+        n = ctypes.c_int64()
+        tick = ctypes.c_double()
+        lib().osy_batch_item_times(self._h, None, 0, ctypes.byref(n), ctypes.byref(tick))
+        arr = (OsyBatchItemTime * max(1, n.value))()
+        check(lib().osy_batch_item_times(self._h, arr, n.value, ctypes.byref(n),
+                                         ctypes.byref(tick)))
+        rec = np.ctypeslib.as_array(arr)[: n.value]
+        out = {f: rec[f].copy() for f in ("member", "kind", "stage", "tile", "hw_id")}
+        t0 = int(rec["claim"].min()) if n.value else 0
+        for f in ("claim", "ready", "done"):
+            out[f] = (rec[f].astype(np.int64) - t0) * tick.value
+        out["tick_ns"] = tick.value
+        return out
 
     @property
     def info(self) -> dict:

@@ -100,6 +100,14 @@ class OsyBatchInfo(ctypes.Structure):
                 ("used_bytes", ctypes.c_uint64)]
 
 
+class OsyBatchItemTime(ctypes.Structure):
+    """struct osy_batch_item_time: one work item of a batch's last step."""
+
+    _fields_ = [("member", _i32), ("kind", _i32), ("stage", _i32), ("tile", _i32),
+                ("claim", ctypes.c_uint64), ("ready", ctypes.c_uint64),
+                ("done", ctypes.c_uint64), ("hw_id", ctypes.c_uint64)]
+
+
 # name -> (restype, argtypes); every function of include/opensayal.h.
 PROTOTYPES = {
     "osy_abi_version": (ctypes.c_int, []),
@@ -146,6 +154,10 @@ PROTOTYPES = {
     "osy_batch_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(OsyBatchTuning),
                                           ctypes.POINTER(ctypes.c_double)]),
     "osy_batch_get_info": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(OsyBatchInfo)]),
+    "osy_batch_set_timing": (ctypes.c_int, [ctypes.c_void_p, _i32]),
+    "osy_batch_item_times": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(OsyBatchItemTime),
+                                            ctypes.c_int64, ctypes.POINTER(ctypes.c_int64),
+                                            ctypes.POINTER(ctypes.c_double)]),
     "osy_stream_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p)]),
     "osy_stream_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "osy_stream_synchronize": (ctypes.c_int, [ctypes.c_void_p]),

@@ -72,6 +72,10 @@ struct BatchArgs {
   int* error;                // set when a dependency wait exceeds spin_ticks
   long long spin_ticks;      // bound of one dependency wait, wall-clock ticks
   int tw, th, k;             // register-tile projection shape
+  // per item, when not null: the wall-clock ticks at which a workgroup
+  // claimed it, found its dependencies complete and finished it, and the
+  // workgroup's __smid() (XCD, shader engine, CU)
+  unsigned long long* stamps;
 };
 
 // Register-tile rows per wavefront supported by the batch kernel, and its

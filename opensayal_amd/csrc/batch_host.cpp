@@ -31,12 +31,17 @@ struct osy_batch {
     BatchArgs args{};
     std::vector<int> member;  // indices into members
     std::vector<int> flip_uv, flip_sm;
+    std::vector<BatchStage> stages;  // host copies of the launch's tables
+    std::vector<uint4> items;
   };
   std::vector<Group> groups;
   void* dev = nullptr;  // every group's tables and counters, and the error word
   int* error = nullptr;
   hipGraphExec_t exec = nullptr;
   long long spin_ticks = 0;
+  double tick_ns = 10.0;  // wall-clock tick
+  bool timing = false;    // stamp every work item (osy_batch_set_timing)
+  bool stamped = false;   // a step ran since the tables were built with stamps
 
   ~osy_batch() { release(); if (stream) (void)hipStreamDestroy(stream); }
   void release() {
@@ -47,6 +52,7 @@ struct osy_batch {
     error = nullptr;
     groups.clear();
     dirty = true;
+    stamped = false;
   }
 };
 
@@ -218,7 +224,8 @@ int rebuild(osy_batch* b) {
              align256(t.stages.size() * sizeof(BatchStage)) +
              align256(t.items.size() * sizeof(uint4)) +
              align256(std::max<size_t>(1, t.deps.size()) * sizeof(unsigned)) + align256(8) +
-             align256(t.stages.size() * 8) + align256(t.items.size() * 8);
+             align256(t.stages.size() * 8) + align256(t.items.size() * 8) +
+             (b->timing ? align256(t.items.size() * 32) : 0);
   }
 
   hipError_t e = hipMalloc(&b->dev, bytes);
@@ -257,6 +264,13 @@ int rebuild(osy_batch* b) {
     p += align256(t.stages.size() * 8);
     a.tdone = (unsigned long long*)p;
     p += align256(t.items.size() * 8);
+    a.stamps = nullptr;
+    if (b->timing) {
+      a.stamps = (unsigned long long*)p;
+      p += align256(t.items.size() * 32);
+    }
+    g.stages = t.stages;
+    g.items = t.items;
     a.nitems = (int)t.items.size();
     a.error = b->error;
     a.spin_ticks = b->spin_ticks;
@@ -298,6 +312,7 @@ int enqueue_steps(osy_batch* b, int32_t steps, hipStream_t s) {
   if (!b->exec || steps == 0) return OSY_OK;
   for (int32_t i = 0; i < steps; i++) OSY_HIP_CHECK(hipGraphLaunch(b->exec, s));
   if (s != b->stream) b->last = s;
+  if (b->timing) b->stamped = true;
   if (steps & 1) {
     for (const osy_batch::Group& g : b->groups)
       for (size_t i = 0; i < g.member.size(); i++) {
@@ -353,6 +368,7 @@ int osy_batch_create(uint64_t budget_bytes, osy_batch** out) {
   }
   // one dependency wait is bounded by 2 s of wall clock
   b->spin_ticks = (long long)(khz > 0 ? khz : 100000) * 2000;
+  b->tick_ns = 1e6 / (khz > 0 ? khz : 100000);
   *out = b;
   return OSY_OK;
 }
@@ -523,6 +539,55 @@ int osy_batch_autotune(osy_batch* b, osy_batch_tuning* chosen, double* ms) {
     chosen->occupancy = b->occupancy;
   }
   if (ms) *ms = best;
+  return OSY_OK;
+}
+
+int osy_batch_set_timing(osy_batch* b, int32_t enable) {
+  if (!b) return set_error(OSY_E_INVALID, "null handle");
+  int st = sync_batch(b);
+  if (st != OSY_OK) return st;
+  if (b->timing != (enable != 0)) {
+    b->timing = enable != 0;
+    b->release();  // the tables are rebuilt with (or without) the stamps
+  }
+  return OSY_OK;
+}
+
+int osy_batch_item_times(osy_batch* b, osy_batch_item_time* out, int64_t capacity,
+                         int64_t* count, double* tick_ns) {
+  if (!b || !count) return set_error(OSY_E_INVALID, "null argument");
+  int st = sync_batch(b);
+  if (st != OSY_OK) return st;
+  if (!b->timing || !b->stamped)
+    return set_error(OSY_E_STATE, "no timed batch step: enable timing, then step");
+  int64_t n = 0;
+  for (const osy_batch::Group& g : b->groups) n += g.args.nitems;
+  *count = n;
+  if (tick_ns) *tick_ns = b->tick_ns;
+  if (capacity < n || (n > 0 && !out))
+    return set_error(OSY_E_INVALID, "the record array holds " + std::to_string(capacity) +
+                                        " items; the last step has " + std::to_string(n));
+  std::vector<unsigned long long> raw;
+  int64_t o = 0;
+  for (const osy_batch::Group& g : b->groups) {
+    raw.resize(4 * (size_t)g.args.nitems);
+    OSY_HIP_CHECK(hipMemcpy(raw.data(), g.args.stamps, raw.size() * 8, hipMemcpyDeviceToHost));
+    // stage index within its member's chain: count the member's earlier stages
+    std::vector<int> chain_pos(g.stages.size(), 0), seen(g.member.size(), 0);
+    for (size_t si = 0; si < g.stages.size(); si++) chain_pos[si] = seen[g.stages[si].sim]++;
+    for (int i = 0; i < g.args.nitems; i++, o++) {
+      const BatchStage& stg = g.stages[g.items[i].x];
+      osy_batch_item_time& r = out[o];
+      r.member = g.member[stg.sim];
+      r.kind = stg.kind;
+      r.stage = chain_pos[g.items[i].x];
+      r.tile = (int32_t)g.items[i].y;
+      r.claim = raw[4 * i];
+      r.ready = raw[4 * i + 1];
+      r.done = raw[4 * i + 2];
+      r.hw_id = raw[4 * i + 3];
+    }
+  }
   return OSY_OK;
 }
 

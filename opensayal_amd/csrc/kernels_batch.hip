@@ -157,7 +157,9 @@ __global__ void __launch_bounds__(kBatchThreads, OCC2 ? 4 : 1) k_batch_step(Batc
     const int sid = __builtin_amdgcn_readfirstlane((int)it.x);
     const int tile = __builtin_amdgcn_readfirstlane((int)it.y);
     const BatchStage st = A.stages[sid];
+    unsigned long long t_claim = 0, t_ready = 0;
     if (threadIdx.x < 64) {  // wavefront 0 waits; thread 0 reports
+      if (A.stamps) t_claim = wall_clock64();
       bool ok = true;
       if (st.tile_dep)
         ok = wait_tiles(A.tdone, A.deps + it.z, (int)it.w, epoch + 1, A.error, A.spin_ticks);
@@ -167,6 +169,7 @@ __global__ void __launch_bounds__(kBatchThreads, OCC2 ? 4 : 1) k_batch_step(Batc
       if (threadIdx.x == 0) {
         if (!ok) __hip_atomic_store(A.error, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         s_ok = ok ? 1 : 0;
+        if (A.stamps) t_ready = wall_clock64();
       }
     }
     // thread 0's acquire invalidated the caches this workgroup reads
@@ -183,6 +186,13 @@ __global__ void __launch_bounds__(kBatchThreads, OCC2 ? 4 : 1) k_batch_step(Batc
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     if (threadIdx.x == 0) {
+      if (A.stamps) {
+        unsigned long long* r = A.stamps + 4 * (size_t)idx;
+        r[0] = t_claim;
+        r[1] = t_ready;
+        r[2] = wall_clock64();
+        r[3] = __smid();
+      }
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
       __hip_atomic_fetch_add(A.done + sid, 1ULL, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       __hip_atomic_fetch_add(A.tdone + idx, 1ULL, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);

@@ -29,7 +29,7 @@ def test_library_exports_every_declared_function():
     for name in names:
         assert hasattr(lib, name), name
         assert name in _abi.PROTOTYPES, name
-    assert lib.osy_abi_version() == 5
+    assert lib.osy_abi_version() == 6
 
 
 def test_struct_sizes_match_header_layout():
@@ -39,6 +39,8 @@ def test_struct_sizes_match_header_layout():
     assert ctypes.sizeof(_abi.OsyBatchTuning) == 3 * 4
     # count, groups, then four 8-byte fields
     assert ctypes.sizeof(_abi.OsyBatchInfo) == 8 + 4 * 8
+    # four 4-byte fields, then four 8-byte fields
+    assert ctypes.sizeof(_abi.OsyBatchItemTime) == 4 * 4 + 4 * 8
 
 
 DOCUMENTED_DEFAULTS = dict(

@@ -249,3 +249,47 @@ def test_batched_workload_256_grids():
                 (k, cfg.width, cfg.height, cfg.projection_n, name)
     for fl in batched + alone:
         fl.close()
+
+
+def test_item_timing_records_every_item_and_changes_nothing():
+    """With per-item timing on, each work item of the last step has one
+    record (member, stage kind, stage, tile; claim <= ready <= done), the
+    items per member and kind are the member's stage chain times its tiles,
+    and every member's fields equal those of the same batch untimed."""
+    cfgs = mixed_configs()
+    timed, plain = pairs(cfgs)
+    with Batch() as bt, Batch() as bp:
+        for fl in timed:
+            bt.add(fl)
+        for fl in plain:
+            bp.add(fl)
+        bt.set_timing(True)
+        with pytest.raises(OpenSayalError):
+            bt.item_times()  # no timed step yet
+        bt.step(2)
+        bp.step(2)
+        rec = bt.item_times()
+        tun = bt.tuning
+        for i, (x, y) in enumerate(zip(timed, plain)):
+            assert_same(state(x), state(y), f"timed batch member {i}")
+        n = len(rec["kind"])
+        assert n == bt.info["work_items"]
+        assert np.all(rec["claim"] <= rec["ready"]) and np.all(rec["ready"] <= rec["done"])
+        assert rec["claim"].min() == 0 and rec["done"].max() > 0
+        k = tun["sweeps_per_tile"]
+        tw, th = 128 - 4 * k, 8 * tun["rows"] - 4 * k
+        for m, c in enumerate(cfgs):
+            mine = rec["member"] == m
+            nproj = -(-c.projection_n // k) if c.projection_n else 0
+            tiles_p = -(-c.width // tw) * -(-c.height // th)
+            tiles_a = -(-c.width // 64) * -(-c.height // 64)
+            assert np.sum(mine & (rec["kind"] == 1)) == nproj * tiles_p, m
+            assert np.sum(mine & (rec["kind"] == 2)) == tiles_a, m
+            smoke_on = c.enable_smoke and c.wind_smoke != 0
+            assert np.sum(mine & (rec["kind"] == 3)) == (tiles_a if smoke_on else 0), m
+            forces = c.projection_n == 0 or not smoke_on
+            assert np.sum(mine & (rec["kind"] == 0)) == (tiles_a if forces else 0), m
+        bt.set_timing(False)
+        bt.step(1)
+        with pytest.raises(OpenSayalError):
+            bt.item_times()
