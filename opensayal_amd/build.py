@@ -35,6 +35,7 @@ SOURCES = [
     "kernels_batch.hip",
     "kernels_view.hip",
     "noalloc.hip",
+    "arith_check.hip",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -259,11 +259,43 @@ template <bool CS1> __device__ inline float centre(int a, int cs) {
   else return (float)(((double)a + 0.5) * (double)cs);
 }
 
-// 1.0 / (distance + 1e-6) in double, rounded to float (src/fluid.cu:690-693),
-// with get_distance (inc/helper.cuh:77-81).
-__device__ inline float inv_distance(float dx, float dy) {
-  return (float)(1.0 / ((double)sqrtf(dx * dx + dy * dy) + 1e-6));
+// 1 / x in double: the hardware reciprocal refined by two Newton-Raphson
+// steps r <- r + r (1 - x r), each a pair of fused multiply-adds.  For x
+// converted from a finite nonzero binary32 value the relative error is at
+// most 2^-50 (exhaustive check on the device: osy_arith_check, mode 1).
+__device__ inline double recip_nr(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = __builtin_fma(-x, r, 1.0);
+  r = __builtin_fma(r, e, r);
+  e = __builtin_fma(-x, r, 1.0);
+  return __builtin_fma(r, e, r);
 }
+
+// 1.0 / (distance + 1e-6) in double, rounded to float (src/fluid.cu:690-693),
+// with get_distance (inc/helper.cuh:77-81).  The quotient is recip_nr's, an
+// infinite distance giving a zero of its sign; rounded to float that equals
+// the rounded IEEE quotient for every binary32 distance, NaN and infinities
+// included (exhaustive check on the device: osy_arith_check, mode 0).
+__device__ inline float inv_from_distance(float s) {
+  const double x = (double)s + 1e-6;
+  const float r = (float)recip_nr(x);
+  return __builtin_isinf(x) ? __builtin_copysignf(0.0f, s) : r;
+}
+
+__device__ inline float inv_distance(float dx, float dy) {
+  return inv_from_distance(sqrtf(dx * dx + dy * dy));
+}
+
+// inv / sum rounded to float (the weights of src/fluid.cu:700-712), given
+// rs = recip_nr(sum): (float)(inv * rs) in double.  Exact: rs is within
+// 2^-50 of 1 / sum and the product rounds once more (2^-53), so the double
+// is within 2^-49 of inv / sum; a quotient of two binary32 numbers A 2^a /
+// (B 2^b) (significands below 2^24) differs from every binary32 rounding
+// midpoint (2M + 1) 2^(e-1) by at least 2^(e-1) / B, more than 2^-49 of
+// itself, so both round to the same float.  (Mode 2 of osy_arith_check
+// compares the two on 2^32 pairs.)  A zero sum (every weight 0) gives NaN,
+// as the division does.
+__device__ inline float idw_ratio(float inv, double rs) { return (float)((double)inv * rs); }
 
 // interpolate_smoke (src/fluid.cu:644-716).
 template <bool CS1, class Valid, class Val>
@@ -286,11 +318,12 @@ __device__ inline float sample_smoke(int cs, float x, float y, const Valid& vali
   const float inv3 = inv_distance(x - x1, y - y3);
   const float inv4 = inv_distance(x - x2, y - y3);
   const float sum_inv = ((inv1 + inv2) + inv3) + inv4;
+  const double rs = recip_nr((double)sum_inv);
   float acc = 0.0f;
-  if (valid(i, j)) acc = acc + (inv1 / sum_inv) * val(i, j);
-  if (valid(i2, j)) acc = acc + (inv2 / sum_inv) * val(i2, j);
-  if (valid(i, j3)) acc = acc + (inv3 / sum_inv) * val(i, j3);
-  if (valid(i2, j3)) acc = acc + (inv4 / sum_inv) * val(i2, j3);
+  if (valid(i, j)) acc = acc + idw_ratio(inv1, rs) * val(i, j);
+  if (valid(i2, j)) acc = acc + idw_ratio(inv2, rs) * val(i2, j);
+  if (valid(i, j3)) acc = acc + idw_ratio(inv3, rs) * val(i, j3);
+  if (valid(i2, j3)) acc = acc + idw_ratio(inv4, rs) * val(i2, j3);
   return acc;
 }
 
@@ -396,11 +429,12 @@ __device__ inline float sample_smoke_bf(int cs, float x, float y, const Valid& v
   const float inv3 = inv_distance(x - x1, y - y3);
   const float inv4 = inv_distance(x - x2, y - y3);
   const float sum_inv = ((inv1 + inv2) + inv3) + inv4;
+  const double rs = recip_nr((double)sum_inv);
   float acc = 0.0f;
-  acc = add_if(acc, valid(i, j), (inv1 / sum_inv) * val(i, j));
-  acc = add_if(acc, valid(i2, j), (inv2 / sum_inv) * val(i2, j));
-  acc = add_if(acc, valid(i, j3), (inv3 / sum_inv) * val(i, j3));
-  acc = add_if(acc, valid(i2, j3), (inv4 / sum_inv) * val(i2, j3));
+  acc = add_if(acc, valid(i, j), idw_ratio(inv1, rs) * val(i, j));
+  acc = add_if(acc, valid(i2, j), idw_ratio(inv2, rs) * val(i2, j));
+  acc = add_if(acc, valid(i, j3), idw_ratio(inv3, rs) * val(i, j3));
+  acc = add_if(acc, valid(i2, j3), idw_ratio(inv4, rs) * val(i2, j3));
   return acc;
 }
 
@@ -423,14 +457,15 @@ __device__ inline float lds_sample_smoke(int cs, float x, float y, const float* 
   const float inv3 = inv_distance(x - x1, y - y3);
   const float inv4 = inv_distance(x - x2, y - y3);
   const float sum_inv = ((inv1 + inv2) + inv3) + inv4;
+  const double rs = recip_nr((double)sum_inv);
   const int b1 = (j - y0) * RX + (i - x0);
   const int b2 = b1 + (left ? -1 : 1);
   const int dj = low ? -RX : RX;
   float acc = 0.0f;
-  acc = add_if(acc, f[b1] & CF_FLUID, (inv1 / sum_inv) * a[b1]);
-  acc = add_if(acc, f[b2] & CF_FLUID, (inv2 / sum_inv) * a[b2]);
-  acc = add_if(acc, f[b1 + dj] & CF_FLUID, (inv3 / sum_inv) * a[b1 + dj]);
-  acc = add_if(acc, f[b2 + dj] & CF_FLUID, (inv4 / sum_inv) * a[b2 + dj]);
+  acc = add_if(acc, f[b1] & CF_FLUID, idw_ratio(inv1, rs) * a[b1]);
+  acc = add_if(acc, f[b2] & CF_FLUID, idw_ratio(inv2, rs) * a[b2]);
+  acc = add_if(acc, f[b1 + dj] & CF_FLUID, idw_ratio(inv3, rs) * a[b1 + dj]);
+  acc = add_if(acc, f[b2 + dj] & CF_FLUID, idw_ratio(inv4, rs) * a[b2 + dj]);
   return acc;
 }
 
