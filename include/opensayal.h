@@ -367,15 +367,17 @@ int osy_event_elapsed_ms(void* start, void* end, float* ms);
  * For scaling hardware byte counters against a known traffic. */
 int osy_calibration_copy(uint64_t bytes, int32_t reps);
 
-/* Exhaustive device checks of the arithmetic the smoke advection's
- * inverse-distance weights use in place of IEEE divisions (no counterpart
- * in the reference; the proof that the results are unchanged, run on the
- * GPU that runs the solver).  Each mode visits all 2^32 binary32 patterns:
+/* Exhaustive device checks of the arithmetic the solver uses in place of
+ * the plain operations (no counterpart in the reference; the proof that the
+ * results are unchanged, run on the GPU that runs the solver).  Each mode
+ * visits all 2^32 binary32 patterns:
  *   0: inv_distance's reciprocal path against 1.0 / (s + 1e-6) in double
  *      rounded to float, bit for bit (src/fluid.cu:690-693);
  *   1: the refined double reciprocal of every finite nonzero value, within
  *      a relative 2^-50;
- *   2: the weights' quotient against IEEE binary32 division, on 2^32 pairs.
+ *   2: the weights' quotient against IEEE binary32 division, on 2^32 pairs;
+ *   3, 4: the projection's paired rounding to FP16 / BF16 storage (one
+ *      packed conversion) against one scalar conversion per value.
  * Writes the number of inputs that fail and the smallest failing pattern
  * (0xFFFFFFFF when none); synchronous. */
 int osy_arith_check(int32_t mode, uint64_t* mismatches, uint32_t* first_input);

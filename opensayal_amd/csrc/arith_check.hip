@@ -1,5 +1,6 @@
 // arith_check.hip - exhaustive on-device checks of the arithmetic shortcuts
-// the smoke advection's inverse-distance weights take (fluid_device.hpp):
+// the smoke advection's inverse-distance weights and the 16-bit storage
+// modes' paired roundings take (fluid_device.hpp):
 // each mode runs every one of the 2^32 binary32 bit patterns through the
 // shortcut and through the IEEE operations it replaces, on the GPU that
 // runs the solver, and counts the inputs where they differ.
@@ -67,6 +68,23 @@ __global__ void __launch_bounds__(kCheckThreads)
         const double e = __builtin_fma(-d, recip_nr(d), 1.0);
         bad = !(__builtin_fabs(e) <= 0x1p-50);
       }
+    } else if (mode == 3 || mode == 4) {
+      // rnd2 (one packed conversion) against rnd (one scalar conversion per
+      // value) for the storage type, each pattern in both halves of a pair
+      const float y = __uint_as_float(bits ^ 0x9E3779B9u);
+      float a = x, b = y, c = y, d = x;
+      float ra, rb, rc, rd;
+      if (mode == 3) {
+        rnd2<__half>(a, b);
+        rnd2<__half>(c, d);
+        ra = rnd<__half>(x), rb = rnd<__half>(y), rc = rb, rd = ra;
+      } else {
+        rnd2<__hip_bfloat16>(a, b);
+        rnd2<__hip_bfloat16>(c, d);
+        ra = rnd<__hip_bfloat16>(x), rb = rnd<__hip_bfloat16>(y), rc = rb, rd = ra;
+      }
+      bad = __float_as_uint(a) != __float_as_uint(ra) || __float_as_uint(b) != __float_as_uint(rb) ||
+            __float_as_uint(c) != __float_as_uint(rc) || __float_as_uint(d) != __float_as_uint(rd);
     } else {
       // the weights' quotient: positive finite sums (and 0) with a
       // numerator no larger; NaN compared as NaN
@@ -88,7 +106,7 @@ using namespace osy;
 
 extern "C" int osy_arith_check(int32_t mode, uint64_t* mismatches, uint32_t* first_input) {
   if (!mismatches || !first_input) return set_error(OSY_E_INVALID, "null argument");
-  if (mode < 0 || mode > 2) return set_error(OSY_E_INVALID, "mode must be 0, 1 or 2");
+  if (mode < 0 || mode > 4) return set_error(OSY_E_INVALID, "mode must be 0 to 4");
   void* buf = nullptr;
   OSY_HIP_CHECK(hipMalloc(&buf, 16));
   unsigned long long* count = (unsigned long long*)buf;

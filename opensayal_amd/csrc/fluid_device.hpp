@@ -66,6 +66,37 @@ template <typename T> __device__ inline float ldf(const T* p, long k) { return S
 // updates (as a launch per phase storing into 16-bit arrays would), so
 // 16-bit results do not depend on how sweeps are grouped into launches.
 template <typename T> __device__ inline float rnd(float x) { return Store<T>::ld(Store<T>::st(x)); }
+
+// rnd of two values with one packed conversion (v_cvt_pk_f16_f32 /
+// v_cvt_pk_bf16_f32, round to nearest even like the scalar conversions;
+// exhaustive check on the device: osy_arith_check, modes 3 and 4).
+typedef float Float2V __attribute__((ext_vector_type(2)));
+template <typename T> struct Round2 {
+  __device__ static void run(float& a, float& b) {
+    a = rnd<T>(a);
+    b = rnd<T>(b);
+  }
+};
+template <> struct Round2<float> {
+  __device__ static void run(float&, float&) {}
+};
+template <> struct Round2<__half> {
+  __device__ static void run(float& a, float& b) {
+    typedef _Float16 H2 __attribute__((ext_vector_type(2)));
+    const H2 h = __builtin_convertvector((Float2V){a, b}, H2);
+    a = (float)h.x;
+    b = (float)h.y;
+  }
+};
+template <> struct Round2<__hip_bfloat16> {
+  __device__ static void run(float& a, float& b) {
+    typedef __bf16 B2 __attribute__((ext_vector_type(2)));
+    const B2 h = __builtin_convertvector((Float2V){a, b}, B2);
+    a = (float)h.x;
+    b = (float)h.y;
+  }
+};
+template <typename T> __device__ inline void rnd2(float& a, float& b) { Round2<T>::run(a, b); }
 template <typename T> __device__ inline void stf(T* p, long k, float x) { p[k] = Store<T>::st(x); }
 
 // Workgroup b of a launch of n runs on XCD b % 8 (workgroups are dispatched

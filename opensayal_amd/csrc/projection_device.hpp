@@ -181,13 +181,19 @@ __device__ __attribute__((always_inline)) inline void reg_row(RegTile<R>& t, con
     if constexpr (PRESSURE) {
       if (g & G_UPD) t.s0[rho] = t.s0[rho] + pressure_term(d, density, csf, d_t);
     }
-    t.u0[rho] = rnd<T>(gated_add(uu, d, g, 0));
-    t.u1[rho] = rnd<T>(gated_sub(ur, d, g, 1));
-    t.v0[rho] = rnd<T>(gated_add(vv, d, g, 2));
-    if (rho + 1 < R)
-      t.v0[up] = rnd<T>(gated_sub(vt, d, g, 3));
-    else
+    float nu0 = gated_add(uu, d, g, 0), nu1 = gated_sub(ur, d, g, 1);
+    rnd2<T>(nu0, nu1);
+    t.u0[rho] = nu0;
+    t.u1[rho] = nu1;
+    if (rho + 1 < R) {
+      float nv = gated_add(vv, d, g, 2), nt = gated_sub(vt, d, g, 3);
+      rnd2<T>(nv, nt);
+      t.v0[rho] = nv;
+      t.v0[up] = nt;
+    } else {
+      t.v0[rho] = rnd<T>(gated_add(vv, d, g, 2));
       dtop = gate_pos(d, g, 3);
+    }
   } else {
     // active cell 2l+1: faces u1, u0 of lane l+1, v1[rho], v1[rho+1]
     const uint32_t g = t.gw[rho] >> 8;
@@ -198,13 +204,19 @@ __device__ __attribute__((always_inline)) inline void reg_row(RegTile<R>& t, con
     if constexpr (PRESSURE) {
       if (g & G_UPD) t.s1[rho] = t.s1[rho] + pressure_term(d, density, csf, d_t);
     }
-    t.u1[rho] = rnd<T>(gated_add(uu, d, g, 0));
-    t.u0[rho] = rnd<T>(t.u0[rho] - lane_from_left(gate_pos(d, g, 1)));
-    t.v1[rho] = rnd<T>(gated_add(vv, d, g, 2));
-    if (rho + 1 < R)
-      t.v1[up] = rnd<T>(gated_sub(vt, d, g, 3));
-    else
+    float nu1 = gated_add(uu, d, g, 0), nu0 = t.u0[rho] - lane_from_left(gate_pos(d, g, 1));
+    rnd2<T>(nu1, nu0);
+    t.u1[rho] = nu1;
+    t.u0[rho] = nu0;
+    if (rho + 1 < R) {
+      float nv = gated_add(vv, d, g, 2), nt = gated_sub(vt, d, g, 3);
+      rnd2<T>(nv, nt);
+      t.v1[rho] = nv;
+      t.v1[up] = nt;
+    } else {
+      t.v1[rho] = rnd<T>(gated_add(vv, d, g, 2));
       dtop = gate_pos(d, g, 3);
+    }
   }
 }
 
