@@ -245,11 +245,12 @@ __global__ void __launch_bounds__(kTileThreads)
   }
 }
 
-// 10 rows per wavefront are held within 128 VGPRs, so two workgroups share
-// a CU as they do at 8 rows; 4 rows within 80, so three share one (the
-// others take what the compiler allocates).
+// 8 and 10 rows per wavefront are held within 128 VGPRs, so two workgroups
+// share a CU (8 rows with pressure sums spill a few values outside the
+// sweep loop, which beats one workgroup per CU); 4 rows within 80, so three
+// share one (the others take what the compiler allocates).
 template <typename T, bool PRESSURE, int R>
-__global__ void __launch_bounds__(64 * kRegWaves, R == 10 ? 4 : (R == 4 ? 6 : 1))
+__global__ void __launch_bounds__(64 * kRegWaves, (R == 8 || R == 10) ? 4 : (R == 4 ? 6 : 1))
     k_project_regs(StepParams sp, const uint8_t* __restrict__ gates, const T* __restrict__ u_in,
                    const T* __restrict__ v_in, T* __restrict__ u_out, T* __restrict__ v_out,
                    float* __restrict__ p, RegArgs a) {
