@@ -334,6 +334,13 @@ int osy_batch_get_tuning(const osy_batch* batch, osy_batch_tuning* tuning);
  * it to `chosen` and its time per batch step to `ms` (either may be NULL). */
 int osy_batch_autotune(osy_batch* batch, osy_batch_tuning* chosen, double* ms);
 int osy_batch_get_info(const osy_batch* batch, osy_batch_info* info);
+/* Members of at least `cells` cells (W*H) fill the GPU on their own: each
+ * batch step runs them through their own per-grid step graphs, on the
+ * batch's stream ahead of the persistent launch, instead of as its work
+ * items (a lone large grid then steps as fast as alone; the results are the
+ * same bits either way).  Default 2^20; 0 keeps every member in the
+ * persistent launch.  No counterpart in the reference. */
+int osy_batch_set_per_grid_cells(osy_batch* batch, int64_t cells);
 /* Per-item timing of the batch's persistent launches (no counterpart in the
  * reference; a profiling aid).  Enabled, every launch stamps each of its
  * work items (four stores per item); the results are unchanged. */

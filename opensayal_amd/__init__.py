@@ -412,6 +412,11 @@ class Batch:
         return {"rows": t.rows, "sweeps_per_tile": t.sweeps_per_tile,
                 "occupancy": t.occupancy}, ms.value
 
+    def set_per_grid_cells(self, cells: int) -> None:
+        """Members of at least `cells` cells step on their own per-grid
+        graphs ahead of the persistent launch (default 2^20; 0 = none)."""
+        check(lib().osy_batch_set_per_grid_cells(self._h, int(cells)))
+
     def set_timing(self, enable: bool = True) -> None:
         """Stamp every work item of every launch (claim, dependencies
         complete, done; results unchanged)."""

@@ -155,6 +155,7 @@ PROTOTYPES = {
                                           ctypes.POINTER(ctypes.c_double)]),
     "osy_batch_get_info": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(OsyBatchInfo)]),
     "osy_batch_set_timing": (ctypes.c_int, [ctypes.c_void_p, _i32]),
+    "osy_batch_set_per_grid_cells": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64]),
     "osy_arith_check": (ctypes.c_int, [_i32, ctypes.POINTER(ctypes.c_uint64),
                                        ctypes.POINTER(ctypes.c_uint32)]),
     "osy_batch_item_times": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(OsyBatchItemTime),

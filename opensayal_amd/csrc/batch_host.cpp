@@ -23,6 +23,10 @@ struct osy_batch {
   // the shape the batch autotuner picks for the wind-tunnel mixes (256 grids
   // of 128..1024 and of 64..384 cells wide): 12 rows, 10 sweeps, two per CU
   int rows = 12, k = 10, occupancy = 2;
+  // members of at least this many cells step through their own per-grid
+  // graphs (osy_batch_set_per_grid_cells); 0 = none
+  int64_t per_grid_cells = int64_t(1) << 20;
+  std::vector<osy_fluid*> solo;  // those members, in membership order
   bool dirty = true;
 
   struct Group {
@@ -51,6 +55,7 @@ struct osy_batch {
     dev = nullptr;
     error = nullptr;
     groups.clear();
+    solo.clear();
     dirty = true;
     stamped = false;
   }
@@ -113,9 +118,14 @@ int rebuild(osy_batch* b) {
   const int rows = b->rows, k = b->k;
   const int tw = 128 - 4 * k, th = 8 * rows - 4 * k;
 
-  // groups by (storage, pressure), in order of first appearance
+  // groups by (storage, pressure), in order of first appearance; members
+  // that fill the GPU alone step on their own graphs instead
   for (size_t m = 0; m < b->members.size(); m++) {
-    const osy_fluid* fl = b->members[m];
+    osy_fluid* fl = b->members[m];
+    if (b->per_grid_cells > 0 && (int64_t)fl->W * fl->H >= b->per_grid_cells) {
+      b->solo.push_back(fl);
+      continue;
+    }
     const bool pressure = fl->cfg.enable_pressure != 0;
     auto it = std::find_if(b->groups.begin(), b->groups.end(), [&](const osy_batch::Group& g) {
       return g.storage == fl->storage && g.pressure == pressure;
@@ -281,6 +291,10 @@ int rebuild(osy_batch* b) {
   }
   OSY_HIP_CHECK(hipStreamSynchronize(b->stream));
 
+  if (b->groups.empty()) {  // every member steps on its own graphs
+    b->dirty = false;
+    return OSY_OK;
+  }
   // one step = one persistent launch per group, captured once
   hipGraph_t graph = nullptr;
   OSY_HIP_CHECK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
@@ -309,8 +323,12 @@ int enqueue_steps(osy_batch* b, int32_t steps, hipStream_t s) {
     if (st == OSY_OK) st = rebuild(b);
     if (st != OSY_OK) return st;
   }
-  if (!b->exec || steps == 0) return OSY_OK;
-  for (int32_t i = 0; i < steps; i++) OSY_HIP_CHECK(hipGraphLaunch(b->exec, s));
+  if ((!b->exec && b->solo.empty()) || steps == 0) return OSY_OK;
+  for (int32_t i = 0; i < steps; i++) {
+    for (osy_fluid* fl : b->solo)
+      if (int st = fluid_enqueue_graph_steps(fl, 1, s)) return st;
+    if (b->exec) OSY_HIP_CHECK(hipGraphLaunch(b->exec, s));
+  }
   if (s != b->stream) b->last = s;
   if (b->timing) b->stamped = true;
   if (steps & 1) {
@@ -539,6 +557,18 @@ int osy_batch_autotune(osy_batch* b, osy_batch_tuning* chosen, double* ms) {
     chosen->occupancy = b->occupancy;
   }
   if (ms) *ms = best;
+  return OSY_OK;
+}
+
+int osy_batch_set_per_grid_cells(osy_batch* b, int64_t cells) {
+  if (!b) return set_error(OSY_E_INVALID, "null handle");
+  if (cells < 0) return set_error(OSY_E_INVALID, "cells must be >= 0");
+  int st = sync_batch(b);
+  if (st != OSY_OK) return st;
+  if (b->per_grid_cells != cells) {
+    b->per_grid_cells = cells;
+    b->release();
+  }
   return OSY_OK;
 }
 

@@ -229,6 +229,10 @@ int field_elem_ok(int field) { return field >= OSY_FIELD_VEL_X && field <= OSY_F
 
 }  // namespace
 
+int osy::fluid_enqueue_graph_steps(osy_fluid* fl, int32_t steps, hipStream_t s) {
+  return enqueue_graph_steps(fl, steps, fl->cfg.d_t, s);
+}
+
 size_t osy::fluid_block_bytes(int W, int H, int storage) {
   return block_layout(W, H, storage).total();
 }

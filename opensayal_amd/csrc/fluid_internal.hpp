@@ -109,4 +109,8 @@ size_t fluid_block_bytes(int W, int H, int storage);
 int batch_wait(osy_fluid* fl);
 void batch_forget(osy_fluid* fl);
 
+// `steps` steps of a handle through its own per-grid step graphs, enqueued
+// on `s` (fluid_host.cpp); a batch runs its large members this way.
+int fluid_enqueue_graph_steps(osy_fluid* fl, int32_t steps, hipStream_t s);
+
 }  // namespace osy

@@ -293,3 +293,33 @@ def test_item_timing_records_every_item_and_changes_nothing():
         bt.step(1)
         with pytest.raises(OpenSayalError):
             bt.item_times()
+
+
+def test_large_members_step_on_their_own_graphs():
+    """Members of at least per_grid_cells cells step through their per-grid
+    graphs ahead of the persistent launch; with them in the mix, alone in
+    the batch, or forced into the persistent launch (0), every member's
+    fields equal its run alone."""
+    big = make_config(1100, 620, projection_n=23, wind_speed=2.0, g=-5.0, pipe_height=150,
+                      obstacle_center_x=400, obstacle_center_y=300, obstacle_radius=30.0)
+    small = mixed_configs()[:4]
+    for cells, cfgs in (((1 << 20) // 2, [big] + small), ((1 << 20) // 2, [big]),
+                        (0, [big] + small)):
+        batched, alone = pairs(cfgs, seed0=300)
+        with Batch() as b:
+            b.set_per_grid_cells(cells)
+            for fl in batched:
+                b.add(fl)
+            b.step(2)
+            b.step(1)
+            info = b.info
+            for fl in alone:
+                fl.step(3)
+            for i, (x, y) in enumerate(zip(batched, alone)):
+                assert_same(state(x), state(y), f"cells={cells} member {i}")
+            if cells and len(cfgs) == 1:
+                assert info["work_items"] == 0 and info["groups"] == 0
+        for fl in batched + alone:
+            fl.close()
+    with Batch() as b, pytest.raises(OpenSayalError):
+        b.set_per_grid_cells(-1)
