@@ -34,7 +34,8 @@ HBM_PEAK_GBS = 8000.0
 # per (kernel, tile width, tile height, sweeps per launch, W, H); summary and
 # commands in profiles/step_1920x1080_n50_fp32.md.
 MEASURED_LAUNCH_TRAFFIC = {
-    (2, 108, 44, 5, 1920, 1080): 38_077_860,
+    (2, 108, 44, 5, 1920, 1080): 38_090_423,
+    (2, 104, 40, 6, 1920, 1080): 38_554_377,
 }
 METRIC = "Gcell-steps/s at 1920x1080 n=50; % HBM roofline; p50/p99 batched step latency"
 
