@@ -35,11 +35,15 @@ __device__ inline float relaxed_quotient(float div, uint32_t g, float o) {
   return o * q;
 }
 
-// The same quotient for the register-resident kernel: `slow` (wavefront
-// uniform) says whether any lane of this row may need the division; only
-// such rows divide, every lane, and select.
-__device__ inline float row_quotient(float div, uint32_t g, float o, bool slow) {
-  const float y = __uint_as_float(0x3F800000u - (((g >> G_L2_SHIFT) & 3u) << 23));
+// 2^-log2(s) of a cell with gate byte g (for s = 1, 2, 4).
+__device__ inline float gate_scale(uint32_t g) {
+  return __uint_as_float(0x3F800000u - (((g >> G_L2_SHIFT) & 3u) << 23));
+}
+
+// The same quotient for the register-resident kernel, from the cell's
+// gate_scale y: `slow` (wavefront uniform) says whether any lane of this
+// row may need the division; only such rows divide, every lane, and select.
+__device__ inline float row_quotient(float div, float y, uint32_t g, float o, bool slow) {
   float q = div * y;
   if (slow) {
     const float qs = div / (float)__builtin_popcount(g & 15u);
@@ -157,9 +161,17 @@ struct RegTile {
   float u0[R], u1[R], v0[R], v1[R];  // faces of cells 2l and 2l+1, rows 0..R-1
   float s0[R], s1[R];                // pressure sums (pressure runs only)
   uint32_t gw[R];                    // gate bytes: cell 2l in bits 0-7, 2l+1 in 8-15
+  float y0[R], y1[R];                // gate_scale of cells 2l and 2l+1 (reg_keeps_scale)
   uint32_t slow;                     // wavefront-uniform: bit 2 rho + c set when some
                                      // lane's cell c of row rho divides (total_s 0 or 3)
 };
+
+// Whether a tile keeps each cell's gate_scale in registers for the launch
+// (two VGPRs per row) rather than deriving it from the gate byte in every
+// phase: where the registers fit without costing occupancy or spilling
+// (8 rows without pressure sums: 120 VGPRs; at 4 rows they would cost a
+// wavefront per SIMD, at 10 and with pressure sums they would spill).
+template <int R, bool PRESSURE> constexpr bool reg_keeps_scale() { return R == 8 && !PRESSURE; }
 
 // apply_projection_at for the active cell of row rho (parity B) of every
 // lane.  Cells near the region edge compute with stale neighbours; their
@@ -177,7 +189,8 @@ __device__ __attribute__((always_inline)) inline void reg_row(RegTile<R>& t, con
     const float uu = t.u0[rho], ur = t.u1[rho], vv = t.v0[rho];
     const float vt = (rho + 1 < R) ? t.v0[up] : vt0;
     const float div = ((ur - uu) + vt) - vv;
-    const float d = row_quotient(div, g, o, (t.slow >> (2 * rho)) & 1u);
+    const float y = reg_keeps_scale<R, PRESSURE>() ? t.y0[rho] : gate_scale(g);
+    const float d = row_quotient(div, y, g, o, (t.slow >> (2 * rho)) & 1u);
     if constexpr (PRESSURE) {
       if (g & G_UPD) t.s0[rho] = t.s0[rho] + pressure_term(d, density, csf, d_t);
     }
@@ -200,7 +213,8 @@ __device__ __attribute__((always_inline)) inline void reg_row(RegTile<R>& t, con
     const float uu = t.u1[rho], ur = lane_from_right(t.u0[rho]), vv = t.v1[rho];
     const float vt = (rho + 1 < R) ? t.v1[up] : vt1;
     const float div = ((ur - uu) + vt) - vv;
-    const float d = row_quotient(div, g, o, (t.slow >> (2 * rho + 1)) & 1u);
+    const float y = reg_keeps_scale<R, PRESSURE>() ? t.y1[rho] : gate_scale(g);
+    const float d = row_quotient(div, y, g, o, (t.slow >> (2 * rho + 1)) & 1u);
     if constexpr (PRESSURE) {
       if (g & G_UPD) t.s1[rho] = t.s1[rho] + pressure_term(d, density, csf, d_t);
     }
@@ -394,6 +408,10 @@ __device__ __attribute__((always_inline)) inline void project_regs_tile(const St
     }
     // the gate bytes of cells 2l and 2l+1 (static, built with the flags)
     t.gw[rho] = (in0 ? (graw[rho] & 0xFFu) : 0u) | (in1 ? (graw[rho] & 0xFF00u) : 0u);
+    if constexpr (reg_keeps_scale<R, PRESSURE>()) {
+      t.y0[rho] = gate_scale(t.gw[rho]);
+      t.y1[rho] = gate_scale(t.gw[rho] >> 8);
+    }
     t.slow |= (__any((t.gw[rho] & G_SLOW) != 0) ? 1u : 0u) << (2 * rho);
     t.slow |= (__any((t.gw[rho] & (G_SLOW << 8)) != 0) ? 1u : 0u) << (2 * rho + 1);
   }
