@@ -35,14 +35,6 @@ __device__ inline float relaxed_quotient(float div, uint32_t g, float o) {
   return o * q;
 }
 
-// div / s for a cell that divides (G_SLOW: total_s 3 or 0) without a
-// division: third_quotient for s = 3, div * +inf for s = 0 (+-inf, or NaN
-// for a zero or NaN div, as div / +0).  Both equal IEEE division for every
-// binary32 div (exhaustive check: tests/csrc/quotient_check.c).
-__device__ inline float slow_quotient(float div, uint32_t g) {
-  return __builtin_popcount(g & 15u) == 3 ? third_quotient(div) : div * __builtin_inff();
-}
-
 // 2^-log2(s) of a cell with gate byte g (for s = 1, 2, 4).
 __device__ inline float gate_scale(uint32_t g) {
   return __uint_as_float(0x3F800000u - (((g >> G_L2_SHIFT) & 3u) << 23));
@@ -53,7 +45,10 @@ __device__ inline float gate_scale(uint32_t g) {
 // row may need the division; only such rows divide, every lane, and select.
 __device__ inline float row_quotient(float div, float y, uint32_t g, float o, bool slow) {
   float q = div * y;
-  if (slow) q = (g & G_SLOW) ? slow_quotient(div, g) : q;
+  if (slow) {
+    const float qs = div / (float)__builtin_popcount(g & 15u);
+    q = (g & G_SLOW) ? qs : q;
+  }
   return o * q;
 }
 
