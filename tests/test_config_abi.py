@@ -138,3 +138,21 @@ def test_create_rejects_invalid_configs_before_device_work(bad, status):
 def test_default_config_helper_sets_overrides():
     c = default_config(64, 36, wind_speed=2.0, storage="f16")
     assert c.width == 64 and c.wind_speed == 2.0 and c.storage == 1 and c.viscosity == 0.0
+
+
+def test_diagnostic_entry_points_reject_bad_arguments_before_device_work():
+    """The timing, per-grid threshold and arithmetic-check entry points
+    validate their arguments before touching a device."""
+    lib = _abi.lib()
+    n = ctypes.c_uint64()
+    first = ctypes.c_uint32()
+    assert lib.osy_arith_check(5, ctypes.byref(n), ctypes.byref(first)) == _abi.OSY_E_INVALID
+    assert b"mode" in lib.osy_last_error()
+    assert lib.osy_arith_check(-1, ctypes.byref(n), ctypes.byref(first)) == _abi.OSY_E_INVALID
+    assert lib.osy_arith_check(0, None, ctypes.byref(first)) == _abi.OSY_E_INVALID
+    assert b"null" in lib.osy_last_error()
+    count = ctypes.c_int64()
+    assert lib.osy_batch_set_timing(None, 1) == _abi.OSY_E_INVALID
+    assert lib.osy_batch_set_per_grid_cells(None, 0) == _abi.OSY_E_INVALID
+    assert lib.osy_batch_item_times(None, None, 0, ctypes.byref(count), None) == \
+        _abi.OSY_E_INVALID

@@ -270,6 +270,14 @@ def test_item_timing_records_every_item_and_changes_nothing():
         bp.step(2)
         rec = bt.item_times()
         tun = bt.tuning
+        # a record array one short is refused, with the count needed
+        import ctypes
+        from opensayal_amd import _abi
+        need = ctypes.c_int64()
+        short = (_abi.OsyBatchItemTime * max(1, len(rec["kind"]) - 1))()
+        assert _abi.lib().osy_batch_item_times(bt._h, short, len(rec["kind"]) - 1,
+                                               ctypes.byref(need), None) == _abi.OSY_E_INVALID
+        assert need.value == len(rec["kind"])
         for i, (x, y) in enumerate(zip(timed, plain)):
             assert_same(state(x), state(y), f"timed batch member {i}")
         n = len(rec["kind"])
