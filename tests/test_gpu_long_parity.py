@@ -76,6 +76,20 @@ def test_full_size_1000_steps_bit_exact():
     assert g[1] == r[1]
 
 
+def test_full_size_pressure_on_bit_exact():
+    """The same grid with the pressure solve on (pressure sums in every
+    sweep, the forces zeroing them): 20 steps bit-exact, pressure field and
+    range included."""
+    cfg = wl.plain_full(pressure=True)
+    fl, o = pair(cfg, *wl.initial_state(cfg, 4))
+    fl.step(20)
+    o.update(steps=20)
+    assert_bitwise(fl, o, "20 steps, pressure on")
+    p = fl.get_field("pressure")
+    assert np.array_equal(p.view(np.uint32), o.p.view(np.uint32))
+    assert fl.pressure_range == o.pressure_range
+
+
 FALLBACK_CASES = [
     # (cell size, velocity noise amplitude): |v| d_t / cs of up to 6.5, 10, 10, 7 cells
     (1, 130.0),
