@@ -384,7 +384,10 @@ int osy_calibration_copy(uint64_t bytes, int32_t reps);
  *      a relative 2^-50;
  *   2: the weights' quotient against IEEE binary32 division, on 2^32 pairs;
  *   3, 4: the projection's paired rounding to FP16 / BF16 storage (one
- *      packed conversion) against one scalar conversion per value.
+ *      packed conversion) against one scalar conversion per value;
+ *   5, 6: the pressure sums' quotient x / d_t as a double multiply by the
+ *      double nearest 1 / d_t, against IEEE division, for d_t = 0.05 and
+ *      d_t = 1/60.
  * Writes the number of inputs that fail and the smallest failing pattern
  * (0xFFFFFFFF when none); synchronous. */
 int osy_arith_check(int32_t mode, uint64_t* mismatches, uint32_t* first_input);

@@ -1,6 +1,7 @@
 // arith_check.hip - exhaustive on-device checks of the arithmetic shortcuts
-// the smoke advection's inverse-distance weights and the 16-bit storage
-// modes' paired roundings take (fluid_device.hpp):
+// the smoke advection's inverse-distance weights, the 16-bit storage
+// modes' paired roundings and the pressure sums take (fluid_device.hpp,
+// projection_device.hpp):
 // each mode runs every one of the 2^32 binary32 bit patterns through the
 // shortcut and through the IEEE operations it replaces, on the GPU that
 // runs the solver, and counts the inputs where they differ.
@@ -68,6 +69,12 @@ __global__ void __launch_bounds__(kCheckThreads)
         const double e = __builtin_fma(-d, recip_nr(d), 1.0);
         bad = !(__builtin_fabs(e) <= 0x1p-50);
       }
+    } else if (mode == 5 || mode == 6) {
+      // the pressure sums' quotient x / d_t as a double multiply by the
+      // double nearest 1 / d_t, for the step sizes 0.05 and 1/60
+      const float b = mode == 5 ? 0.05f : 1.0f / 60.0f;
+      const float q1 = (float)((double)x * (1.0 / (double)b));
+      bad = __float_as_uint(q1) != __float_as_uint(div_ieee(x, b));
     } else if (mode == 3 || mode == 4) {
       // rnd2 (one packed conversion) against rnd (one scalar conversion per
       // value) for the storage type, each pattern in both halves of a pair
@@ -106,7 +113,7 @@ using namespace osy;
 
 extern "C" int osy_arith_check(int32_t mode, uint64_t* mismatches, uint32_t* first_input) {
   if (!mismatches || !first_input) return set_error(OSY_E_INVALID, "null argument");
-  if (mode < 0 || mode > 4) return set_error(OSY_E_INVALID, "mode must be 0 to 4");
+  if (mode < 0 || mode > 6) return set_error(OSY_E_INVALID, "mode must be 0 to 6");
   void* buf = nullptr;
   OSY_HIP_CHECK(hipMalloc(&buf, 16));
   unsigned long long* count = (unsigned long long*)buf;

@@ -95,6 +95,7 @@ struct osy_fluid {
     sp.visc_on = c.viscosity != 0.0f;
     sp.visc_a = c.viscosity * d_t / (float)(cs * cs);
     sp.visc_den = 1.0f + 4.0f * sp.visc_a;
+    sp.pdt_recip = (d_t != 0.0f && std::fabs(d_t) < 2.0f) ? 1.0 / (double)d_t : 0.0;
     return sp;
   }
 };

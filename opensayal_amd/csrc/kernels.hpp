@@ -44,6 +44,9 @@ struct StepParams {
   // set by the step: the first projection launch applies the velocity
   // forces to what it loads and starts the pressure sums at 0
   int forces_in_projection;
+  // 1.0 / (double)d_t when 0 < |d_t| < 2, else 0: the pressure sums'
+  // division by d_t as a double multiply (PressureDivisor)
+  double pdt_recip;
 };
 
 // Device buffers of one simulation; T is the storage type of u, v, smoke.

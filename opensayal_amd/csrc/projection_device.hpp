@@ -118,6 +118,13 @@ __device__ inline float lane_from_left(float x) {  // lane l gets lane l-1's x (
   return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x138, 0xf, 0xf, true));
 }
 
+// gate of d: d when the bit is set, else -0 (which an addition leaves out
+// bit for bit)
+__device__ inline float gate_neg(float d, uint32_t g, int bit) {
+  const uint32_t m = (uint32_t)((int32_t)(g << (31 - bit)) >> 31);
+  return __uint_as_float((__float_as_uint(d) & m) | (~m & 0x80000000u));
+}
+
 // gate of d: d when the bit is set, else +0
 __device__ inline float gate_pos(float d, uint32_t g, int bit) {
   const uint32_t m = (uint32_t)((int32_t)(g << (31 - bit)) >> 31);
@@ -150,9 +157,25 @@ struct RegArgs {
 
 __device__ inline bool lc_in_tile(int lc, int K, int tw) { return lc >= 2 * K && lc < 2 * K + tw; }
 
-// update_pressure_at (src/fluid.cu:221-227): ((d * density) * cell_size) / d_t
-__device__ inline float pressure_term(float d, float density, float csf, float d_t) {
-  return ((d * density) * csf) / d_t;
+// The divisor of update_pressure_at: d_t, and recip = 1.0 / (double)d_t
+// (the double nearest 1/d_t) when 0 < |d_t| < 2, else 0.
+struct PressureDivisor {
+  float d_t;
+  double recip;
+};
+
+// update_pressure_at (src/fluid.cu:221-227): ((d * density) * cell_size) / d_t.
+// With recip set the quotient is (float)(x * recip) in double, which is the
+// IEEE float quotient x / d_t for every binary32 x: the double is within
+// 2^-52 of x / d_t; a quotient of two binary32 numbers lies at least 2^-49
+// of itself from every binary32 rounding midpoint in the normal range, and
+// in the subnormal range it can only equal a midpoint if the divisor's
+// lowest set bit weighs 2 or more, which |d_t| < 2 excludes.  (Exhaustive
+// device check for d_t = 0.05 and 1/60: osy_arith_check, modes 5 and 6.)
+__device__ inline float pressure_term(float d, float density, float csf,
+                                      const PressureDivisor& pd) {
+  const float x = (d * density) * csf;
+  return pd.recip != 0.0 ? (float)((double)x * pd.recip) : x / pd.d_t;
 }
 
 // Register state of one lane of the register-resident kernel.
@@ -181,7 +204,7 @@ template <typename T, int R, int B, bool PRESSURE>
 __device__ __attribute__((always_inline)) inline void reg_row(RegTile<R>& t, const int rho,
                                                               float vt0, float vt1, float& dtop,
                                                               float o, float density, float csf,
-                                                              float d_t) {
+                                                              const PressureDivisor& pd) {
   const int up = rho + 1 < R ? rho + 1 : rho;  // row above (when inside the strip)
   if constexpr (B == 0) {
     // active cell 2l: faces u0, u1, v0[rho], v0[rho+1] (all in this lane)
@@ -192,7 +215,8 @@ __device__ __attribute__((always_inline)) inline void reg_row(RegTile<R>& t, con
     const float y = reg_keeps_scale<R, PRESSURE>() ? t.y0[rho] : gate_scale(g);
     const float d = row_quotient(div, y, g, o, (t.slow >> (2 * rho)) & 1u);
     if constexpr (PRESSURE) {
-      if (g & G_UPD) t.s0[rho] = t.s0[rho] + pressure_term(d, density, csf, d_t);
+      // a cell that is not projected adds -0, which leaves its sum's bits
+      t.s0[rho] = t.s0[rho] + gate_neg(pressure_term(d, density, csf, pd), g, 4);
     }
     float nu0 = gated_add(uu, d, g, 0), nu1 = gated_sub(ur, d, g, 1);
     rnd2<T>(nu0, nu1);
@@ -216,7 +240,7 @@ __device__ __attribute__((always_inline)) inline void reg_row(RegTile<R>& t, con
     const float y = reg_keeps_scale<R, PRESSURE>() ? t.y1[rho] : gate_scale(g);
     const float d = row_quotient(div, y, g, o, (t.slow >> (2 * rho + 1)) & 1u);
     if constexpr (PRESSURE) {
-      if (g & G_UPD) t.s1[rho] = t.s1[rho] + pressure_term(d, density, csf, d_t);
+      t.s1[rho] = t.s1[rho] + gate_neg(pressure_term(d, density, csf, pd), g, 4);
     }
     float nu1 = gated_add(uu, d, g, 0), nu0 = t.u0[rho] - lane_from_left(gate_pos(d, g, 1));
     rnd2<T>(nu1, nu0);
@@ -246,10 +270,10 @@ template <typename T, int R, int PB, bool PRESSURE, int RHO>
 __device__ __attribute__((always_inline)) inline void reg_rows_from(RegTile<R>& t, float vt0,
                                                                     float vt1, float& dtop,
                                                                     float o, float density,
-                                                                    float csf, float d_t) {
+                                                                    float csf, const PressureDivisor& pd) {
   if constexpr (RHO < R) {
-    reg_row<T, R, ((PB + RHO) & 1), PRESSURE>(t, RHO, vt0, vt1, dtop, o, density, csf, d_t);
-    reg_rows_from<T, R, PB, PRESSURE, RHO + 1>(t, vt0, vt1, dtop, o, density, csf, d_t);
+    reg_row<T, R, ((PB + RHO) & 1), PRESSURE>(t, RHO, vt0, vt1, dtop, o, density, csf, pd);
+    reg_rows_from<T, R, PB, PRESSURE, RHO + 1>(t, vt0, vt1, dtop, o, density, csf, pd);
   }
 }
 
@@ -283,7 +307,7 @@ __device__ __attribute__((always_inline)) inline void reg_phase(RegTile<R>& t, R
                                                                 int w, int rbase, int s,
                                                                 int nphase, int K, int th,
                                                                 float o, float density,
-                                                                float csf, float d_t) {
+                                                                float csf, const PressureDivisor& pd) {
   const int e = nphase - 1 - s;
   const int lo = 2 * K - 1 - e;
   const int rhi = 2 * K + th + e;
@@ -306,7 +330,7 @@ __device__ __attribute__((always_inline)) inline void reg_phase(RegTile<R>& t, R
     // phase loop into registers
 #pragma unroll
     for (int rho = 0; rho < R; rho++) asm volatile("" : "+v"(t.gw[rho]));
-    reg_rows_from<T, R, PB, PRESSURE, 0>(t, vt0, vt1, dtop, o, density, csf, d_t);
+    reg_rows_from<T, R, PB, PRESSURE, 0>(t, vt0, vt1, dtop, o, density, csf, pd);
   }
   if constexpr (ONE_SYNC) {
     if (w > 0) sh.vrow[b][w][lane] = make_float2(t.v0[0], t.v1[0]);
@@ -418,6 +442,7 @@ __device__ __attribute__((always_inline)) inline void project_regs_tile(const St
 
   const float o = sp.o;
   const float csf = (float)sp.cs;
+  const PressureDivisor pdiv{sp.d_t, sp.pdt_recip};
   const int nphase = 2 * a.sweeps;
   float vt0 = 0.0f, vt1 = 0.0f;
   if constexpr (ONE_SYNC) {  // row 0 of the wavefront above, as loaded
@@ -434,16 +459,16 @@ __device__ __attribute__((always_inline)) inline void project_regs_tile(const St
   if (((gy0 + rbase) & 1) == 0) {
     for (int s = 0; s < nphase; s += 2) {
       reg_phase<T, R, 0, PRESSURE, ONE_SYNC>(t, sh, vt0, vt1, lane, w, rbase, s, nphase, K, a.th, o,
-                                   sp.density, csf, sp.d_t);
+                                   sp.density, csf, pdiv);
       reg_phase<T, R, 1, PRESSURE, ONE_SYNC>(t, sh, vt0, vt1, lane, w, rbase, s + 1, nphase, K, a.th, o,
-                                   sp.density, csf, sp.d_t);
+                                   sp.density, csf, pdiv);
     }
   } else {
     for (int s = 0; s < nphase; s += 2) {
       reg_phase<T, R, 1, PRESSURE, ONE_SYNC>(t, sh, vt0, vt1, lane, w, rbase, s, nphase, K, a.th, o,
-                                   sp.density, csf, sp.d_t);
+                                   sp.density, csf, pdiv);
       reg_phase<T, R, 0, PRESSURE, ONE_SYNC>(t, sh, vt0, vt1, lane, w, rbase, s + 1, nphase, K, a.th, o,
-                                   sp.density, csf, sp.d_t);
+                                   sp.density, csf, pdiv);
     }
   }
 

@@ -20,7 +20,9 @@ pytestmark = pytest.mark.gpu
                                         (1, "reciprocal within 2^-50"),
                                         (2, "weight quotient vs IEEE division"),
                                         (3, "paired FP16 rounding vs scalar"),
-                                        (4, "paired BF16 rounding vs scalar")])
+                                        (4, "paired BF16 rounding vs scalar"),
+                                        (5, "pressure quotient, d_t = 0.05"),
+                                        (6, "pressure quotient, d_t = 1/60")])
 def test_arith_shortcut_exhaustive(mode, what):
     n = ctypes.c_uint64()
     first = ctypes.c_uint32()
