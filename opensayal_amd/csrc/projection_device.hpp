@@ -184,17 +184,55 @@ struct RegTile {
   float u0[R], u1[R], v0[R], v1[R];  // faces of cells 2l and 2l+1, rows 0..R-1
   float s0[R], s1[R];                // pressure sums (pressure runs only)
   uint32_t gw[R];                    // gate bytes: cell 2l in bits 0-7, 2l+1 in 8-15
-  float y0[R], y1[R];                // gate_scale of cells 2l and 2l+1 (reg_keeps_scale)
+  uint32_t m0[R], m1[R];             // face masks of cells 2l and 2l+1 (reg_face_masks)
   uint32_t slow;                     // wavefront-uniform: bit 2 rho + c set when some
-                                     // lane's cell c of row rho divides (total_s 0 or 3)
+                                     // lane's cell c of row rho needs the general
+                                     // quotient (row_quotient, mask_quotient)
 };
 
-// Whether a tile keeps each cell's gate_scale in registers for the launch
-// (two VGPRs per row) rather than deriving it from the gate byte in every
-// phase: where the registers fit without costing occupancy or spilling
-// (8 rows without pressure sums: 120 VGPRs; at 4 rows they would cost a
-// wavefront per SIMD, at 10 and with pressure sums they would spill).
-template <int R, bool PRESSURE> constexpr bool reg_keeps_scale() { return R == 8 && !PRESSURE; }
+// Whether a tile keeps each cell's four face gates as byte masks (one VGPR
+// per cell, byte k 0xFF when face bit k is set, else 0) rather than as bits
+// of the gate byte: a gated face update is then one AND that selects and
+// sign-extends the mask byte in its operand, and one add.  Pressure runs
+// keep the gate byte, whose projected bit gates the sum.
+template <int R, bool PRESSURE> constexpr bool reg_face_masks() { return !PRESSURE; }
+
+// The face masks of gate byte g: bit k of g (k = 0..3) spread to bit 8k by
+// one multiply (the 16 partial products land on distinct bits), then to
+// the whole byte.
+__device__ inline uint32_t face_masks(uint32_t g) {
+  return (((g & 15u) * 0x00204081u) & 0x01010101u) * 0xFFu;
+}
+
+// d where face K of the masks m is open, else +0.
+template <int K>
+__device__ inline float mask_face(float d, uint32_t m) {
+  const uint32_t mk = (uint32_t)(int32_t)(int8_t)(uint8_t)(m >> (8 * K));
+  return __uint_as_float(__float_as_uint(d) & mk);
+}
+
+// div / s for a cell with face masks m (popcount(m) = 8s).  A cell with
+// s = 0 moves no face, so its quotient is never used, and almost every
+// other cell has s = 4: a row whose lanes all have s = 0 or 4 takes
+// div * 0.25.  The rest (`mixed`, wavefront uniform) take div * 2^-log2(s)
+// for s = 1, 2, 4 (log2(s) = 8s >> 4), exact, and the IEEE quotient for
+// s = 3.
+__device__ inline float mask_quotient(float div, uint32_t m, bool mixed) {
+  float q = div * 0.25f;
+  if (mixed) {
+    const uint32_t c8 = (uint32_t)__builtin_popcount(m);
+    const float qp = __builtin_amdgcn_ldexpf(div, -(int)(c8 >> 4));
+    const float qs = div / (float)(c8 >> 3);
+    q = c8 == 24u ? qs : qp;
+  }
+  return q;
+}
+
+// Whether some lane's face masks m need mask_quotient's general branch.
+__device__ inline bool mask_mixed(uint32_t m) {
+  const uint32_t c8 = (uint32_t)__builtin_popcount(m);
+  return __any(c8 != 0u && c8 != 32u);
+}
 
 // apply_projection_at for the active cell of row rho (parity B) of every
 // lane.  Cells near the region edge compute with stale neighbours; their
@@ -206,14 +244,58 @@ __device__ __attribute__((always_inline)) inline void reg_row(RegTile<R>& t, con
                                                               float o, float density, float csf,
                                                               const PressureDivisor& pd) {
   const int up = rho + 1 < R ? rho + 1 : rho;  // row above (when inside the strip)
-  if constexpr (B == 0) {
+  if constexpr (reg_face_masks<R, PRESSURE>()) {
+    // x + gate(d) as x - mask(-d) and x - gate(d) as x - mask(d): an open
+    // face gets x + d (x - (-d) is x + d in IEEE arithmetic), a closed one
+    // x - (+0), which is x for every x
+    if constexpr (B == 0) {
+      const uint32_t m = t.m0[rho];
+      const float uu = t.u0[rho], ur = t.u1[rho], vv = t.v0[rho];
+      const float vt = (rho + 1 < R) ? t.v0[up] : vt0;
+      const float div = ((ur - uu) + vt) - vv;
+      const float q = mask_quotient(div, m, (t.slow >> (2 * rho)) & 1u);
+      const float d = o * q, nd = -d;
+      float nu0 = uu - mask_face<0>(nd, m), nu1 = ur - mask_face<1>(d, m);
+      rnd2<T>(nu0, nu1);
+      t.u0[rho] = nu0;
+      t.u1[rho] = nu1;
+      if (rho + 1 < R) {
+        float nv = vv - mask_face<2>(nd, m), nt = vt - mask_face<3>(d, m);
+        rnd2<T>(nv, nt);
+        t.v0[rho] = nv;
+        t.v0[up] = nt;
+      } else {
+        t.v0[rho] = rnd<T>(vv - mask_face<2>(nd, m));
+        dtop = mask_face<3>(d, m);
+      }
+    } else {
+      const uint32_t m = t.m1[rho];
+      const float uu = t.u1[rho], ur = lane_from_right(t.u0[rho]), vv = t.v1[rho];
+      const float vt = (rho + 1 < R) ? t.v1[up] : vt1;
+      const float div = ((ur - uu) + vt) - vv;
+      const float q = mask_quotient(div, m, (t.slow >> (2 * rho + 1)) & 1u);
+      const float d = o * q, nd = -d;
+      float nu1 = uu - mask_face<0>(nd, m), nu0 = t.u0[rho] - lane_from_left(mask_face<1>(d, m));
+      rnd2<T>(nu1, nu0);
+      t.u1[rho] = nu1;
+      t.u0[rho] = nu0;
+      if (rho + 1 < R) {
+        float nv = vv - mask_face<2>(nd, m), nt = vt - mask_face<3>(d, m);
+        rnd2<T>(nv, nt);
+        t.v1[rho] = nv;
+        t.v1[up] = nt;
+      } else {
+        t.v1[rho] = rnd<T>(vv - mask_face<2>(nd, m));
+        dtop = mask_face<3>(d, m);
+      }
+    }
+  } else if constexpr (B == 0) {
     // active cell 2l: faces u0, u1, v0[rho], v0[rho+1] (all in this lane)
     const uint32_t g = t.gw[rho];
     const float uu = t.u0[rho], ur = t.u1[rho], vv = t.v0[rho];
     const float vt = (rho + 1 < R) ? t.v0[up] : vt0;
     const float div = ((ur - uu) + vt) - vv;
-    const float y = reg_keeps_scale<R, PRESSURE>() ? t.y0[rho] : gate_scale(g);
-    const float d = row_quotient(div, y, g, o, (t.slow >> (2 * rho)) & 1u);
+    const float d = row_quotient(div, gate_scale(g), g, o, (t.slow >> (2 * rho)) & 1u);
     if constexpr (PRESSURE) {
       // a cell that is not projected adds -0, which leaves its sum's bits
       t.s0[rho] = t.s0[rho] + gate_neg(pressure_term(d, density, csf, pd), g, 4);
@@ -237,8 +319,7 @@ __device__ __attribute__((always_inline)) inline void reg_row(RegTile<R>& t, con
     const float uu = t.u1[rho], ur = lane_from_right(t.u0[rho]), vv = t.v1[rho];
     const float vt = (rho + 1 < R) ? t.v1[up] : vt1;
     const float div = ((ur - uu) + vt) - vv;
-    const float y = reg_keeps_scale<R, PRESSURE>() ? t.y1[rho] : gate_scale(g);
-    const float d = row_quotient(div, y, g, o, (t.slow >> (2 * rho + 1)) & 1u);
+    const float d = row_quotient(div, gate_scale(g), g, o, (t.slow >> (2 * rho + 1)) & 1u);
     if constexpr (PRESSURE) {
       t.s1[rho] = t.s1[rho] + gate_neg(pressure_term(d, density, csf, pd), g, 4);
     }
@@ -329,7 +410,14 @@ __device__ __attribute__((always_inline)) inline void reg_phase(RegTile<R>& t, R
     // compiler keeps it from hoisting the 8 face masks per row out of the
     // phase loop into registers
 #pragma unroll
-    for (int rho = 0; rho < R; rho++) asm volatile("" : "+v"(t.gw[rho]));
+    for (int rho = 0; rho < R; rho++) {
+      if constexpr (reg_face_masks<R, PRESSURE>()) {
+        asm volatile("" : "+v"(t.m0[rho]));
+        asm volatile("" : "+v"(t.m1[rho]));
+      } else {
+        asm volatile("" : "+v"(t.gw[rho]));
+      }
+    }
     reg_rows_from<T, R, PB, PRESSURE, 0>(t, vt0, vt1, dtop, o, density, csf, pd);
   }
   if constexpr (ONE_SYNC) {
@@ -432,12 +520,15 @@ __device__ __attribute__((always_inline)) inline void project_regs_tile(const St
     }
     // the gate bytes of cells 2l and 2l+1 (static, built with the flags)
     t.gw[rho] = (in0 ? (graw[rho] & 0xFFu) : 0u) | (in1 ? (graw[rho] & 0xFF00u) : 0u);
-    if constexpr (reg_keeps_scale<R, PRESSURE>()) {
-      t.y0[rho] = gate_scale(t.gw[rho]);
-      t.y1[rho] = gate_scale(t.gw[rho] >> 8);
+    if constexpr (reg_face_masks<R, PRESSURE>()) {
+      t.m0[rho] = face_masks(t.gw[rho]);
+      t.m1[rho] = face_masks(t.gw[rho] >> 8);
+      t.slow |= (mask_mixed(t.m0[rho]) ? 1u : 0u) << (2 * rho);
+      t.slow |= (mask_mixed(t.m1[rho]) ? 1u : 0u) << (2 * rho + 1);
+    } else {
+      t.slow |= (__any((t.gw[rho] & G_SLOW) != 0) ? 1u : 0u) << (2 * rho);
+      t.slow |= (__any((t.gw[rho] & (G_SLOW << 8)) != 0) ? 1u : 0u) << (2 * rho + 1);
     }
-    t.slow |= (__any((t.gw[rho] & G_SLOW) != 0) ? 1u : 0u) << (2 * rho);
-    t.slow |= (__any((t.gw[rho] & (G_SLOW << 8)) != 0) ? 1u : 0u) << (2 * rho + 1);
   }
 
   const float o = sp.o;
@@ -472,15 +563,20 @@ __device__ __attribute__((always_inline)) inline void project_regs_tile(const St
     }
   }
 
-  // write back the owned tile
-  const bool own0 = lc_in_tile(2 * lane, K, a.tw) && col0;
-  const bool own1 = lc_in_tile(2 * lane + 1, K, a.tw) && col1;
+  // write back the owned tile (lane and column recomputed from a fresh
+  // thread index, so that nothing of the load is kept live over the sweeps)
+  int tid_s = (int)threadIdx.x;
+  asm volatile("" : "+v"(tid_s));
+  const int lane_s = tid_s & 63;
+  const int gi_s = gx0 + 2 * lane_s;
+  const bool own0 = lc_in_tile(2 * lane_s, K, a.tw) && (unsigned)gi_s < (unsigned)W;
+  const bool own1 = lc_in_tile(2 * lane_s + 1, K, a.tw) && (unsigned)(gi_s + 1) < (unsigned)W;
 #pragma unroll
   for (int rho = 0; rho < R; rho++) {
     const int r = rbase + rho;
     const int gj = gy0 + r;
     if (r < 2 * K || r >= 2 * K + a.th || gj >= H) continue;
-    const long gk = (long)gj * P + gi;
+    const long gk = (long)gj * P + gi_s;
     store_pair(u_out, gk, own0, own1, t.u0[rho], t.u1[rho]);
     store_pair(v_out, gk, own0, own1, t.v0[rho], t.v1[rho]);
     if constexpr (PRESSURE) {
