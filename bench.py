@@ -477,15 +477,21 @@ def main() -> int:
     value = W * H * args.steps / (gpu_ms / 1e3) / 1e9
 
     # dominant kernel: kernel_reps projection launches of sweeps_per_launch
-    # sweeps back to back, timed with events on the stream they run on
-    spl = fl.sweeps_per_launch
+    # sweeps back to back (the resident kernel: kernel_reps launches of all n
+    # sweeps), timed with events on the stream they run on
+    resident = fl.projection_kernel == 3
+    spl = n if resident else fl.sweeps_per_launch
     fl.project(sweeps=spl, stream=stream, sync=False)
     a, b = Event(), Event()
     a.record(stream)
-    fl.project(sweeps=spl * args.kernel_reps, stream=stream, sync=False)
+    if resident:
+        for _ in range(args.kernel_reps):
+            fl.project(sweeps=spl, stream=stream, sync=False)
+    else:
+        fl.project(sweeps=spl * args.kernel_reps, stream=stream, sync=False)
     b.record(stream)
     stream.synchronize()
-    launches = fl.launches_per_call(spl * args.kernel_reps)
+    launches = args.kernel_reps if resident else fl.launches_per_call(spl * args.kernel_reps)
     per_launch_ms = a.elapsed_ms(b) / launches
     alg_bytes_per_launch = 17 * spl * args.kernel_reps * W * H / launches
     achieved = alg_bytes_per_launch / (per_launch_ms / 1e3) / 1e9

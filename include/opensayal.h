@@ -123,7 +123,10 @@ typedef struct osy_fluid_info {
   int32_t pitch;        /* elements per device row                        */
   int32_t storage;      /* osy_storage of u, v, smoke                      */
   int32_t projection_kernel; /* 0: one launch per colour phase;
-                                1: LDS-tiled, several sweeps per launch    */
+                                1: LDS-tiled, several sweeps per launch;
+                                2: register tiles, several sweeps per launch;
+                                3: register tiles resident for all sweeps,
+                                   one launch                              */
   int32_t sweeps_per_launch; /* sweeps one projection launch runs          */
   size_t device_bytes;  /* device memory held by this handle               */
 } osy_fluid_info;
@@ -132,10 +135,15 @@ typedef struct osy_fluid_info {
  * reference's thread.cuda.block_size_x/y has no counterpart here).  Results
  * are bit-identical for every valid shape; only speed changes. */
 typedef struct osy_tuning {
-  int32_t projection_kernel; /* 0: one launch per colour phase; 1: tiled    */
+  int32_t projection_kernel; /* 0: one launch per colour phase; 1: tiled;
+                                2: register tiles; 3: register tiles
+                                resident for all sweeps (one launch; every
+                                tile's workgroup must fit on the device at
+                                once, and both tile sides be >= 2 x k)     */
   int32_t tile_width;        /* tiled: cells a workgroup owns along i (even) */
   int32_t tile_height;       /* tiled: cells a workgroup owns along j       */
-  int32_t sweeps_per_launch; /* tiled: sweeps per launch (halo = 2x this)   */
+  int32_t sweeps_per_launch; /* tiled: sweeps per launch (halo = 2x this);
+                                3: sweeps between halo exchanges           */
 } osy_tuning;
 
 typedef struct osy_fluid osy_fluid;

@@ -185,12 +185,15 @@ class Fluid:
     @property
     def projection_kernel_name(self) -> str:
         return {0: "projection_colour", 1: "projection_lds_tiled",
-                2: "projection_register_tiled"}.get(self.projection_kernel, "?")
+                2: "projection_register_tiled",
+                3: "projection_register_resident"}.get(self.projection_kernel, "?")
 
     def launches_per_call(self, sweeps: int) -> int:
         """Projection kernel launches one ``project(sweeps)`` call makes."""
         if self.projection_kernel == 0:
             return 2 * sweeps
+        if self.projection_kernel == 3:
+            return 1 if sweeps > 0 else 0
         k = self.sweeps_per_launch
         return -(-sweeps // k)
 

@@ -73,8 +73,8 @@ size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 // storage type (u, v, smoke, double-buffered), pressure, a reference-order
 // staging buffer, the cell flags and gate bytes, and reduction scratch.
 struct BlockLayout {
-  size_t fld, f32, ref, flg, red, dred;
-  size_t total() const { return 6 * fld + f32 + ref + 2 * flg + red + dred; }
+  size_t fld, f32, ref, flg, red, dred, rsd;
+  size_t total() const { return 6 * fld + f32 + ref + 2 * flg + red + dred + rsd; }
 };
 
 BlockLayout block_layout(int W, int H, int storage) {
@@ -88,6 +88,8 @@ BlockLayout block_layout(int W, int H, int storage) {
   L.flg = align_up(cells, 256);
   L.red = align_up(sizeof(float) * (2 * kReduceBlocks + 2), 256);
   L.dred = align_up(sizeof(double) * (2 * kReduceBlocks + 2), 256);
+  // resident projection: flag words, finished count, error word
+  L.rsd = align_up(sizeof(unsigned) * (kResidentMaxTiles + 2), 256);
   return L;
 }
 
@@ -133,6 +135,13 @@ int create_impl(const osy_config* cfg, osy_fluid** out) {
 
   const BlockLayout L = block_layout(fl->W, fl->H, fl->storage);
   const size_t fld = L.fld, f32 = L.f32, ref = L.ref, flg = L.flg, red = L.red, dred = L.dred;
+  int khz = 0;
+  e = hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, fl->device);
+  if (e != hipSuccess) {
+    delete fl;
+    return set_error(OSY_E_HIP, std::string("device attribute: ") + hipGetErrorString(e));
+  }
+  fl->spin_ticks = (long long)(khz > 0 ? khz : 100000) * 2000;
   fl->block_bytes = L.total();
   e = hipMalloc(&fl->block, fl->block_bytes);
   if (e != hipSuccess) {
@@ -154,6 +163,9 @@ int create_impl(const osy_config* cfg, osy_fluid** out) {
   fl->f.scratch = (float*)b; b += red;
   fl->f.dscratch = (double*)b; b += dred;
   fl->f.prange = fl->f.scratch + 2 * kReduceBlocks;
+  fl->f.rflags = (unsigned*)b;
+  fl->f.error = (int*)(fl->f.rflags + kResidentMaxTiles + 1);
+  b += L.rsd;
 
   e = init_projection_kernels();
   if (e == hipSuccess) e = hipMemsetAsync(fl->block, 0, fl->block_bytes, fl->stream);
@@ -237,6 +249,35 @@ size_t osy::fluid_block_bytes(int W, int H, int storage) {
   return block_layout(W, H, storage).total();
 }
 
+namespace osy {
+
+bool tuning_fits(const osy_fluid* fl, const ProjTuning& t) {
+  if (!t.valid(fl->cfg.enable_pressure != 0)) return false;
+  if (t.kernel != PROJ_RESIDENT) return true;
+  const int tiles = t.tiles(fl->W, fl->H);
+  int slots = 0;
+  if (resident_capacity(fl->storage, fl->cfg.enable_pressure != 0, t.reg_rows(), &slots) !=
+      hipSuccess)
+    return false;
+  return tiles <= kResidentMaxTiles && tiles <= slots;
+}
+
+int fluid_check_error(osy_fluid* fl) {
+  if (fl->tuning.kernel != PROJ_RESIDENT) return OSY_OK;
+  int err = 0;
+  OSY_HIP_CHECK(hipMemcpy(&err, fl->f.error, sizeof(int), hipMemcpyDeviceToHost));
+  if (err) {
+    const int zero = 0;
+    OSY_HIP_CHECK(hipMemcpy(fl->f.error, &zero, sizeof(int), hipMemcpyHostToDevice));
+    return set_error(OSY_E_STATE,
+                     "projection: a neighbour wait of the resident kernel timed out; the "
+                     "fields are invalid");
+  }
+  return OSY_OK;
+}
+
+}  // namespace osy
+
 extern "C" {
 
 int osy_abi_version(void) { return OSY_ABI_VERSION; }
@@ -290,6 +331,7 @@ int osy_fluid_destroy(osy_fluid* fluid) {
   return OSY_OK;
 }
 
+
 int osy_fluid_update(osy_fluid* fl, const osy_source* src, float d_t) {
   if (!fl) return set_error(OSY_E_INVALID, "null handle");
   if (fl->batch) return set_error(OSY_E_STATE, "the handle is stepped by a batch");
@@ -298,14 +340,14 @@ int osy_fluid_update(osy_fluid* fl, const osy_source* src, float d_t) {
                                 : enqueue_graph_steps(fl, 1, d_t, fl->stream);
   if (st != OSY_OK) return st;
   OSY_HIP_CHECK(hipStreamSynchronize(fl->stream));  // src/fluid.cu:794
-  return OSY_OK;
+  return fluid_check_error(fl);
 }
 
 int osy_fluid_step(osy_fluid* fl, int32_t steps, float d_t) {
   int st = osy_fluid_step_async(fl, steps, d_t, nullptr);
   if (st != OSY_OK) return st;
   OSY_HIP_CHECK(hipStreamSynchronize(fl->stream));
-  return OSY_OK;
+  return fluid_check_error(fl);
 }
 
 int osy_fluid_step_async(osy_fluid* fl, int32_t steps, float d_t, void* hip_stream) {
@@ -320,7 +362,7 @@ int osy_fluid_step_async(osy_fluid* fl, int32_t steps, float d_t, void* hip_stre
 int osy_fluid_synchronize(osy_fluid* fl) {
   if (!fl) return set_error(OSY_E_INVALID, "null handle");
   OSY_HIP_CHECK(hipStreamSynchronize(fl->stream));
-  return OSY_OK;
+  return fluid_check_error(fl);
 }
 
 int osy_fluid_project(osy_fluid* fl, int32_t sweeps, void* hip_stream) {
@@ -346,7 +388,7 @@ int osy_fluid_get_field(osy_fluid* fl, int32_t field, void* dst, size_t bytes) {
   OSY_HIP_CHECK(hipGetLastError());
   OSY_HIP_CHECK(hipMemcpyAsync(dst, fl->staging, bytes, hipMemcpyDeviceToHost, fl->stream));
   OSY_HIP_CHECK(hipStreamSynchronize(fl->stream));
-  return OSY_OK;
+  return fluid_check_error(fl);
 }
 
 int osy_fluid_set_field(osy_fluid* fl, int32_t field, const void* src, size_t bytes) {
@@ -397,7 +439,7 @@ int osy_fluid_render_rgba(osy_fluid* fl, uint32_t* dst, size_t bytes) {
   OSY_HIP_CHECK(hipGetLastError());
   OSY_HIP_CHECK(hipMemcpyAsync(dst, fl->staging, bytes, hipMemcpyDeviceToHost, fl->stream));
   OSY_HIP_CHECK(hipStreamSynchronize(fl->stream));
-  return OSY_OK;
+  return fluid_check_error(fl);
 }
 
 int osy_fluid_traces(osy_fluid* fl, int32_t distance, int32_t length, float d_t,
@@ -459,12 +501,19 @@ int osy_fluid_set_tuning(osy_fluid* fl, const osy_tuning* t) {
   if (pt.kernel == PROJ_COLOUR) {
     pt = ProjTuning();
     pt.kernel = PROJ_COLOUR;
+  } else if (pt.kernel == PROJ_RESIDENT && pt.valid(fl->cfg.enable_pressure != 0) &&
+             !tuning_fits(fl, pt)) {
+    return set_error(OSY_E_INVALID,
+                     "resident projection: " + std::to_string(pt.tiles(fl->W, fl->H)) +
+                         " tiles do not all fit on the device at once");
   } else if (!pt.valid(fl->cfg.enable_pressure != 0)) {
     return set_error(OSY_E_INVALID,
                      "tiled projection needs 1 <= k <= 64, an even tile width >= 2, a tile "
                      "height >= 1, a region of at most " +
                          std::to_string(2 * 64 * kTileWaves) +
-                         " cells per row and at most 160 KiB of LDS");
+                         " cells per row and at most 160 KiB of LDS; register and resident "
+                         "tiles a width of 128 - 4k and a region height of 8 x 4, 8, 10, "
+                         "12 or 16 rows (resident: both tile sides >= 2k)");
   }
   OSY_HIP_CHECK(hipStreamSynchronize(fl->stream));
   for (auto& row : fl->graphs)
@@ -499,8 +548,9 @@ TuneKey tune_key(const osy_fluid* fl) {
 }
 
 // Every launch shape the autotuner times: the colour-phase kernel, the
-// register-resident kernel at 4, 8, 10, 12 and 16 rows per wavefront and 1..12
-// sweeps per launch, and a few LDS-tiled shapes.
+// register-resident kernels (several launches, and one resident launch when
+// the tiles fit) at 4, 8, 10, 12 and 16 rows per wavefront and 1..12 sweeps
+// per launch or stage, and a few LDS-tiled shapes.
 std::vector<ProjTuning> tuning_candidates(bool pressure) {
   std::vector<ProjTuning> out;
   ProjTuning c;
@@ -508,12 +558,14 @@ std::vector<ProjTuning> tuning_candidates(bool pressure) {
   out.push_back(c);
   for (int rows : {4, 8, 10, 12, 16})
     for (int k : {1, 2, 3, 4, 5, 6, 7, 8, 10, 12}) {
-      ProjTuning r;
-      r.kernel = PROJ_REGS;
-      r.k = k;
-      r.tw = 128 - 4 * k;
-      r.th = kRegWaves * rows - 4 * k;
-      if (r.valid(pressure)) out.push_back(r);
+      for (int kern : {PROJ_REGS, PROJ_RESIDENT}) {
+        ProjTuning r;
+        r.kernel = kern;
+        r.k = k;
+        r.tw = 128 - 4 * k;
+        r.th = kRegWaves * rows - 4 * k;
+        if (r.valid(pressure)) out.push_back(r);
+      }
     }
   const int lds_shapes[][3] = {{112, 16, 3}, {112, 16, 4}, {176, 24, 4}, {240, 24, 4}};
   for (const auto& sh : lds_shapes) {
@@ -589,6 +641,7 @@ int osy_fluid_autotune(osy_fluid* fl, osy_tuning* chosen, double* ms) {
     best_ms = 1e30;
     for (const ProjTuning& cand : tuning_candidates(pressure)) {
       if (e != hipSuccess) break;
+      if (!tuning_fits(fl, cand)) continue;
       float t_min = 1e30f;
       for (int rep = 0; rep < 3 && e == hipSuccess; rep++) {  // rep 0 warms up
         e = restore(fl->stream);

@@ -35,6 +35,7 @@ struct osy_fluid {
   void* trace_buf = nullptr;  // path-line output, grown on demand
   size_t trace_cap = 0;
   ProjTuning tuning;  // projection kernel shape, chosen at create
+  long long spin_ticks = 0;  // 2 s of the device's wall clock
   osy_batch* batch = nullptr;  // the batch stepping this handle, if any
   struct Graph {
     hipGraphExec_t exec = nullptr;
@@ -96,6 +97,7 @@ struct osy_fluid {
     sp.visc_a = c.viscosity * d_t / (float)(cs * cs);
     sp.visc_den = 1.0f + 4.0f * sp.visc_a;
     sp.pdt_recip = (d_t != 0.0f && std::fabs(d_t) < 2.0f) ? 1.0 / (double)d_t : 0.0;
+    sp.spin_ticks = spin_ticks;
     return sp;
   }
 };
@@ -104,6 +106,15 @@ namespace osy {
 
 // Bytes of the single device allocation of a handle of this shape.
 size_t fluid_block_bytes(int W, int H, int storage);
+
+// Whether the handle can run projection shape t: valid for its pressure
+// mode and, for the resident kernel, every tile's workgroup resident at
+// once on its device.
+bool tuning_fits(const osy_fluid* fl, const ProjTuning& t);
+
+// Reads and clears the device error word (a resident projection whose
+// neighbour wait timed out); call with the stream synchronized.
+int fluid_check_error(osy_fluid* fl);
 
 // Batch membership hooks (batch_host.cpp): wait for the batch's queued
 // steps, and leave the batch (when the handle is destroyed).

@@ -47,6 +47,8 @@ struct StepParams {
   // 1.0 / (double)d_t when 0 < |d_t| < 2, else 0: the pressure sums'
   // division by d_t as a double multiply (PressureDivisor)
   double pdt_recip;
+  // bound of one neighbour wait of the resident projection (wall-clock ticks)
+  long long spin_ticks;
 };
 
 // Device buffers of one simulation; T is the storage type of u, v, smoke.
@@ -60,6 +62,9 @@ struct FieldPtrs {
   float* scratch;   // reductions: partial results
   float* prange;    // [min, max] of pressure
   double* dscratch; // divergence reduction partials
+  unsigned* rflags; // resident projection: stages each tile has published,
+                    // then the count of finished workgroups (kResidentMaxTiles + 1)
+  int* error;       // set by a resident projection whose neighbour wait timed out
 };
 
 // Projection kernels selectable per handle.
@@ -67,7 +72,12 @@ enum ProjectionKernel : int {
   PROJ_COLOUR = 0,  // one launch per colour phase, 2n launches, in place
   PROJ_TILED = 1,   // LDS tiles, several sweeps per launch, ping-pong
   PROJ_REGS = 2,    // register-resident tiles, several sweeps per launch
+  PROJ_RESIDENT = 3,  // register-resident tiles kept for all n sweeps, one launch
 };
+
+// Tiles a resident projection may have (its flag words); a shape also needs
+// every tile's workgroup resident on the GPU at once.
+constexpr int kResidentMaxTiles = 4096;
 
 // Register-resident kernel: 8 wavefronts stacked vertically, each holding
 // 128 cells x kRegRows rows in VGPRs; the region is 128 x 8*kRegRows cells.
@@ -108,17 +118,24 @@ struct ProjTuning {
   int reg_rows() const { return (th + 4 * k) / kRegWaves; }
   bool valid(bool pressure) const {
     if (kernel == PROJ_COLOUR) return true;
-    if (kernel == PROJ_REGS)  // pressure sums need 2 more VGPRs per row: 16 rows spill
-      return k >= 1 && k <= 28 && tw == 128 - 4 * k && th >= 1 &&
-             (th + 4 * k) % kRegWaves == 0 && reg_rows_supported(reg_rows()) &&
-             !(pressure && reg_rows() > 8);
+    if (kernel == PROJ_REGS || kernel == PROJ_RESIDENT) {
+      // pressure sums need 2 more VGPRs per row: 16 rows spill
+      const bool regs = k >= 1 && k <= 28 && tw == 128 - 4 * k && th >= 1 &&
+                        (th + 4 * k) % kRegWaves == 0 && reg_rows_supported(reg_rows()) &&
+                        !(pressure && reg_rows() > 8);
+      // resident: a halo of 2k reaches only the adjacent tiles
+      return regs && (kernel == PROJ_REGS || (tw >= 2 * k && th >= 2 * k));
+    }
     return kernel == PROJ_TILED && k >= 1 && k <= 64 && tw >= 2 && th >= 1 && tw % 2 == 0 &&
            pw() <= 64 * kTileWaves && lds_bytes(pressure) <= kLdsLimit;
   }
   // projection launches for `sweeps` sweeps
   int launches(int sweeps) const {
+    if (kernel == PROJ_RESIDENT) return sweeps > 0 ? 1 : 0;
     return kernel == PROJ_COLOUR ? 2 * sweeps : (sweeps + k - 1) / k;
   }
+  // register kernels: tiles over a W x H grid
+  int tiles(int W, int H) const { return ((W + tw - 1) / tw) * ((H + th - 1) / th); }
 };
 
 // Number of float slots `scratch` must hold, and doubles for `dscratch`.

@@ -283,6 +283,155 @@ void launch_tiled(const StepParams& sp, const GridView& g, const FieldPtrs& f, i
   }
 }
 
+// ---- resident kernel ---------------------------------------------------------
+//
+// All n sweeps in one launch: one workgroup per tile keeps its region in
+// registers from the first sweep to the last.  The sweeps run in stages of
+// k (the last stage takes the rest); after a stage the owned cells are
+// exact and the halo is stale, so the workgroup stores the ring of owned
+// cells its neighbours read as halo (those within 2k of the tile's edge),
+// publishes the stage on its flag, waits for the adjacent tiles' flags and
+// reloads only its halo from their rings.  Stage s reads buffer s & 1 and
+// writes buffer (s + 1) & 1, so a ring is overwritten (two stages later)
+// only after every neighbour has finished the stage that read it, which it
+// did before publishing the stage this tile waited for.  The last stage
+// stores the whole tile (and the pressure sums) into buffer nstages & 1.
+// The last workgroup to finish clears the flags for the next launch.
+//
+// Every tile's workgroup must be resident at once (the host checks the
+// tile count against the occupancy); a wait is bounded by spin_ticks, after
+// which the error word is set and the launch drains without waiting.
+struct ResidentArgs {
+  int nstages;      // ceil(n / k)
+  int last_sweeps;  // sweeps of the last stage
+  int nty;          // tile rows
+  unsigned* flags;  // [tiles]: stages published; [tiles]: finished workgroups
+  int* error;
+  long long spin_ticks;
+};
+
+// Wavefront 0: wait until the adjacent tiles (up to 8, one per lane) have
+// published `stage`.  False on timeout, or at once when another wait of the
+// launch has timed out.  The rings are read with agent-coherent loads
+// issued after this wait (behind the workgroup barrier that follows it), so
+// no cache needs invalidating.
+__device__ inline bool wait_neighbours(const ResidentArgs& ra, int ntx, int tile, unsigned stage) {
+  const int lane = (int)(threadIdx.x & 63);
+  const int tx = tile % ntx, ty = tile / ntx;
+  const int nx = tx + lane % 3 - 1, ny = ty + lane / 3 - 1;
+  const bool poll = lane < 9 && lane != 4 && (unsigned)nx < (unsigned)ntx &&
+                    (unsigned)ny < (unsigned)ra.nty;
+  const unsigned* f = ra.flags + (poll ? ny * ntx + nx : 0);
+  const long long t0 = wall_clock64();
+  for (;;) {
+    const bool ready =
+        !poll || __hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= stage;
+    if (__all(ready)) break;
+    if (__hip_atomic_load(ra.error, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return false;
+    if (wall_clock64() - t0 > ra.spin_ticks) return false;
+    __builtin_amdgcn_s_sleep(1);
+  }
+  return true;
+}
+
+template <typename T, bool PRESSURE, int R>
+__global__ void __launch_bounds__(64 * kRegWaves, (R == 8 || R == 10) ? 4 : (R == 4 ? 6 : 1))
+    k_project_resident(StepParams sp, const uint8_t* __restrict__ gates, T* u0, T* v0, T* u1,
+                       T* v1, float* __restrict__ p, RegArgs a, ResidentArgs ra) {
+  __shared__ RegShared sh;
+  __shared__ int s_last;
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const RegPlace pl = reg_place<R>(a, tile);
+  RegTile<R> t;
+  reg_load<T, PRESSURE, R>(t, sp, gates, u0, v0, p, a, pl);
+  for (int s = 0; s < ra.nstages; s++) {
+    const bool odd = (s & 1) != 0;
+    if (s > 0) {
+      if (threadIdx.x < 64) {
+        const bool ok = wait_neighbours(ra, a.ntx, tile, (unsigned)s);
+        if (!ok && threadIdx.x == 0)
+          __hip_atomic_store(ra.error, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+      // the barrier orders every wavefront's halo loads after the wait
+      __syncthreads();
+      reg_reload_halo<T, R>(t, odd ? u1 : u0, odd ? v1 : v0, sp, a, pl);
+    }
+    const bool last = s + 1 == ra.nstages;
+    reg_sweeps<T, PRESSURE, R, false>(t, sh, sp, pl, a.th, last ? ra.last_sweeps : a.k);
+    T* uo = odd ? u0 : u1;
+    T* vo = odd ? v0 : v1;
+    if (!last) {
+      reg_store<T, PRESSURE, R, true>(t, uo, vo, p, sp, a, pl);
+      // every wavefront's ring stores (agent-coherent) have completed before
+      // the barrier, so they are visible GPU-wide before the flag is
+      __builtin_amdgcn_s_waitcnt(0);
+      __syncthreads();
+      if (threadIdx.x == 0)
+        __hip_atomic_store(ra.flags + tile, (unsigned)(s + 1), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      reg_store<T, PRESSURE, R, false>(t, uo, vo, p, sp, a, pl);
+    }
+  }
+  // every other workgroup has finished waiting when the last one counts
+  // itself in, so that one may clear the flags
+  const int ntiles = a.ntx * ra.nty;
+  if (threadIdx.x == 0)
+    s_last = __hip_atomic_fetch_add(ra.flags + ntiles, 1u, __ATOMIC_RELAXED,
+                                    __HIP_MEMORY_SCOPE_AGENT) == (unsigned)(ntiles - 1);
+  __syncthreads();
+  if (s_last) {
+    for (int i = (int)threadIdx.x; i <= ntiles; i += (int)blockDim.x)
+      __hip_atomic_store(ra.flags + i, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+template <typename T, bool PRESSURE, int R>
+const void* resident_kernel_r() {
+  return (const void*)k_project_resident<T, PRESSURE, R>;
+}
+
+template <typename T, bool PRESSURE>
+const void* resident_kernel(int rows) {
+  switch (rows) {
+    case 4: return resident_kernel_r<T, PRESSURE, 4>();
+    case 8: return resident_kernel_r<T, PRESSURE, 8>();
+    case 10: return resident_kernel_r<T, PRESSURE, 10>();
+    case 12: return resident_kernel_r<T, PRESSURE, 12>();
+    default: return resident_kernel_r<T, PRESSURE, 16>();
+  }
+}
+
+template <typename T, bool PRESSURE>
+void launch_resident(const StepParams& sp, const FieldPtrs& f, int* cur, const ProjTuning& t,
+                     long long spin_ticks, hipStream_t s) {
+  if (sp.n <= 0) return;
+  RegArgs a;
+  a.tw = t.tw;
+  a.th = t.th;
+  a.k = t.k;
+  a.ntx = (sp.W + a.tw - 1) / a.tw;
+  a.sweeps = t.k;
+  a.forces = sp.forces_in_projection ? 1 : 0;
+  ResidentArgs ra;
+  ra.nstages = (sp.n + t.k - 1) / t.k;
+  ra.last_sweeps = sp.n - (ra.nstages - 1) * t.k;
+  ra.nty = (sp.H + a.th - 1) / a.th;
+  ra.flags = f.rflags;
+  ra.error = f.error;
+  ra.spin_ticks = spin_ticks;
+  const int in = *cur, other = 1 - in;
+  T* u0 = (T*)f.u[in];
+  T* v0 = (T*)f.v[in];
+  T* u1 = (T*)f.u[other];
+  T* v1 = (T*)f.v[other];
+  void* args[] = {(void*)&sp, (void*)&f.gates, (void*)&u0, (void*)&v0, (void*)&u1,
+                  (void*)&v1, (void*)&f.p, (void*)&a, (void*)&ra};
+  (void)hipLaunchKernel(resident_kernel<T, PRESSURE>(t.reg_rows()), dim3(a.ntx * ra.nty),
+                        dim3(64 * kRegWaves), args, 0, s);
+  if (ra.nstages & 1) *cur = other;
+}
+
 template <typename T, bool PRESSURE, int R>
 void launch_regs_r(const StepParams& sp, const GridView& g, const FieldPtrs& f, int* cur,
                    const ProjTuning& t, hipStream_t s) {
@@ -326,6 +475,20 @@ hipError_t allow_large_lds() {
   return e;
 }
 
+template <typename T>
+hipError_t resident_capacity_t(bool pressure, int rows, int* slots) {
+  int dev = 0, cus = 0, per_cu = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (e == hipSuccess)
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        &per_cu,
+        pressure ? resident_kernel<T, true>(rows) : resident_kernel<T, false>(rows),
+        64 * kRegWaves, 0);
+  *slots = e == hipSuccess ? per_cu * cus : 0;
+  return e;
+}
+
 }  // namespace
 
 hipError_t init_projection_kernels() {
@@ -335,9 +498,24 @@ hipError_t init_projection_kernels() {
   return e;
 }
 
+hipError_t resident_capacity(int storage, bool pressure, int rows, int* slots) {
+  switch (storage) {
+    case OSY_STORAGE_F16: return resident_capacity_t<__half>(pressure, rows, slots);
+    case OSY_STORAGE_BF16: return resident_capacity_t<__hip_bfloat16>(pressure, rows, slots);
+    default: return resident_capacity_t<float>(pressure, rows, slots);
+  }
+}
+
 template <typename T>
 void enqueue_projection(const StepParams& sp, const GridView& g, const FieldPtrs& f, int* cur,
                         const ProjTuning& t, hipStream_t s) {
+  if (t.kernel == PROJ_RESIDENT && t.valid(sp.pressure_on)) {
+    if (sp.pressure_on)
+      launch_resident<T, true>(sp, f, cur, t, sp.spin_ticks, s);
+    else
+      launch_resident<T, false>(sp, f, cur, t, sp.spin_ticks, s);
+    return;
+  }
   if (t.kernel == PROJ_REGS && t.valid(sp.pressure_on)) {
     if (sp.pressure_on)
       launch_regs<T, true>(sp, g, f, cur, t, s);

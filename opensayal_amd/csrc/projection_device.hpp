@@ -3,6 +3,8 @@
 // (kernels_batch.hip).
 #pragma once
 
+#include <type_traits>
+
 #include "fluid_device.hpp"
 #include "kernels.hpp"
 #include "step_device.hpp"
@@ -94,6 +96,49 @@ __device__ inline void store_pair(T* __restrict__ a, long k, bool in0, bool in1,
   } else {
     if (in0) stf(a, k, x0);
     if (in1) stf(a, k + 1, x1);
+  }
+}
+
+// Accesses of the resident projection's halo exchange, coherent across the
+// GPU without cache maintenance: relaxed agent-scope atomics, which the
+// compiler issues as loads and stores that bypass the caches that are not
+// coherent between CUs and XCDs (a CU's L1, an XCD's L2).  The bits of a
+// pair travel as one 2-element word.
+template <typename T>
+__device__ inline void coherent_load_pair(const T* a, long k, float& x0, float& x1) {
+  using W = typename std::conditional<sizeof(T) == 4, unsigned long long, unsigned>::type;
+  const W w = __hip_atomic_load(reinterpret_cast<const W*>(a + k), __ATOMIC_RELAXED,
+                                __HIP_MEMORY_SCOPE_AGENT);
+  typename PairOf<T>::type pr;
+  __builtin_memcpy(&pr, &w, sizeof(W));
+  x0 = Store<T>::ld(pr.x);
+  x1 = Store<T>::ld(pr.y);
+}
+
+template <typename T>
+__device__ inline void coherent_store_one(T* a, long k, float x) {
+  using W = typename std::conditional<sizeof(T) == 4, unsigned, unsigned short>::type;
+  const T e = Store<T>::st(x);
+  W w;
+  __builtin_memcpy(&w, &e, sizeof(W));
+  __hip_atomic_store(reinterpret_cast<W*>(a + k), w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <typename T>
+__device__ inline void coherent_store_pair(T* a, long k, bool in0, bool in1, float x0,
+                                           float x1) {
+  if (in0 && in1) {  // k is even and rows start on 64-element boundaries
+    using W = typename std::conditional<sizeof(T) == 4, unsigned long long, unsigned>::type;
+    typename PairOf<T>::type pr;
+    pr.x = Store<T>::st(x0);
+    pr.y = Store<T>::st(x1);
+    W w;
+    __builtin_memcpy(&w, &pr, sizeof(W));
+    __hip_atomic_store(reinterpret_cast<W*>(a + k), w, __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+  } else {
+    if (in0) coherent_store_one(a, k, x0);
+    if (in1) coherent_store_one(a, k + 1, x1);
   }
 }
 
@@ -448,39 +493,55 @@ __device__ __attribute__((always_inline)) inline void reg_phase(RegTile<R>& t, R
   }
 }
 
-// One launch of the register-resident projection for tile `tile` (row-major
-// over the grid's ntx tiles per row), run by a workgroup of 64 * kRegWaves
-// threads.
-template <typename T, bool PRESSURE, int R, bool ONE_SYNC>
-__device__ __attribute__((always_inline)) inline void project_regs_tile(const StepParams& sp, const uint8_t* __restrict__ gates,
-                                         const T* __restrict__ u_in, const T* __restrict__ v_in,
-                                         T* __restrict__ u_out, T* __restrict__ v_out,
-                                         float* __restrict__ p, const RegArgs& a, int tile,
-                                         RegShared& sh) {
-  int tid = (int)threadIdx.x;
-  asm volatile("" : "+v"(tid));  // re-read per tile, not kept live across a persistent loop
-  const int lane = tid & 63;
-  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int K = a.k;
-  const int tx = tile % a.ntx, ty = tile / a.ntx;
-  const int x0 = tx * a.tw, y0 = ty * a.th;
-  const int gx0 = x0 - 2 * K, gy0 = y0 - 2 * K;  // region origin; gx0 is even
-  const int W = sp.W, H = sp.H, P = sp.P;
-  const int rbase = w * R;  // first region row of this wavefront
-  const int gi = gx0 + 2 * lane;
-  const bool col0 = (unsigned)gi < (unsigned)W, col1 = (unsigned)(gi + 1) < (unsigned)W;
+// Where a workgroup's tile lies: K (halo 2K), the region origin (gx0 is
+// even) and this wavefront's first region row.  Wavefront-uniform.
+struct RegPlace {
+  int K, gx0, gy0, w, rbase;
+};
 
-  RegTile<R> t;
+template <int R>
+__device__ inline RegPlace reg_place(const RegArgs& a, int tile) {
+  RegPlace pl;
+  pl.K = a.k;
+  const int tx = tile % a.ntx, ty = tile / a.ntx;
+  pl.gx0 = tx * a.tw - 2 * a.k;
+  pl.gy0 = ty * a.th - 2 * a.k;
+  pl.w = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+  pl.rbase = pl.w * R;
+  return pl;
+}
+
+// This lane's index, re-read from the thread index so that it is not kept
+// live across the sweeps.
+__device__ inline int reg_lane() {
+  int tid = (int)threadIdx.x;
+  asm volatile("" : "+v"(tid));
+  return tid & 63;
+}
+
+// Loads the region of u, v (and the pressure sums) and the gate bytes of
+// the tile into registers: every load first, from the even column clamped
+// into the row (so a pair stays inside its row of P cells) and the row
+// clamped into the grid, with no branch, so a wavefront waits for HBM once
+// rather than once per row.  Cells outside the grid are then set to 0.
+template <typename T, bool PRESSURE, int R>
+__device__ __attribute__((always_inline)) inline void reg_load(RegTile<R>& t, const StepParams& sp,
+                                                               const uint8_t* __restrict__ gates,
+                                                               const T* __restrict__ u_in,
+                                                               const T* __restrict__ v_in,
+                                                               const float* __restrict__ p,
+                                                               const RegArgs& a,
+                                                               const RegPlace& pl) {
+  const int lane = reg_lane();
+  const int W = sp.W, H = sp.H, P = sp.P;
+  const int gi = pl.gx0 + 2 * lane;
+  const bool col0 = (unsigned)gi < (unsigned)W, col1 = (unsigned)(gi + 1) < (unsigned)W;
   t.slow = 0;
-  // Every load of the tile first, from the even column clamped into the row
-  // (so a pair stays inside its row of P cells) and the row clamped into the
-  // grid, with no branch: a wavefront waits for HBM once rather than once
-  // per row.  Cells outside the grid are then set to 0.
   const int gic = min(max(gi, 0), P - 2);
   uint32_t graw[R];
 #pragma unroll
   for (int rho = 0; rho < R; rho++) {
-    const long k = (long)min(max(gy0 + rbase + rho, 0), H - 1) * P + gic;
+    const long k = (long)min(max(pl.gy0 + pl.rbase + rho, 0), H - 1) * P + gic;
     load_pair(u_in, k, true, true, t.u0[rho], t.u1[rho]);
     load_pair(v_in, k, true, true, t.v0[rho], t.v1[rho]);
     graw[rho] = *reinterpret_cast<const uint16_t*>(gates + k);
@@ -493,7 +554,7 @@ __device__ __attribute__((always_inline)) inline void project_regs_tile(const St
   __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
   for (int rho = 0; rho < R; rho++) {
-    const int gj = gy0 + rbase + rho;
+    const int gj = pl.gy0 + pl.rbase + rho;
     const bool row_in = (unsigned)gj < (unsigned)H;
     const bool in0 = row_in && col0, in1 = row_in && col1;
     t.u0[rho] = in0 ? t.u0[rho] : 0.0f;
@@ -530,11 +591,22 @@ __device__ __attribute__((always_inline)) inline void project_regs_tile(const St
       t.slow |= (__any((t.gw[rho] & (G_SLOW << 8)) != 0) ? 1u : 0u) << (2 * rho + 1);
     }
   }
+}
 
+// `sweeps` sweeps over the registers: colour 0 (even cells) first; row 0's
+// active parity in a colour-0 phase is (gy0 + rbase) & 1 and flips every
+// phase.
+template <typename T, bool PRESSURE, int R, bool ONE_SYNC>
+__device__ __attribute__((always_inline)) inline void reg_sweeps(RegTile<R>& t, RegShared& sh,
+                                                                 const StepParams& sp,
+                                                                 const RegPlace& pl, int th,
+                                                                 int sweeps) {
+  const int lane = reg_lane();
+  const int w = pl.w, rbase = pl.rbase, K = pl.K;
   const float o = sp.o;
   const float csf = (float)sp.cs;
   const PressureDivisor pdiv{sp.d_t, sp.pdt_recip};
-  const int nphase = 2 * a.sweeps;
+  const int nphase = 2 * sweeps;
   float vt0 = 0.0f, vt1 = 0.0f;
   if constexpr (ONE_SYNC) {  // row 0 of the wavefront above, as loaded
     if (w > 0) sh.vrow[0][w][lane] = make_float2(t.v0[0], t.v1[0]);
@@ -545,45 +617,127 @@ __device__ __attribute__((always_inline)) inline void project_regs_tile(const St
       vt1 = x.y;
     }
   }
-  // colour 0 (even cells) first; row 0's active parity in a colour-0 phase
-  // is (gy0 + rbase) & 1 and flips every phase
-  if (((gy0 + rbase) & 1) == 0) {
+  if (((pl.gy0 + rbase) & 1) == 0) {
     for (int s = 0; s < nphase; s += 2) {
-      reg_phase<T, R, 0, PRESSURE, ONE_SYNC>(t, sh, vt0, vt1, lane, w, rbase, s, nphase, K, a.th, o,
-                                   sp.density, csf, pdiv);
-      reg_phase<T, R, 1, PRESSURE, ONE_SYNC>(t, sh, vt0, vt1, lane, w, rbase, s + 1, nphase, K, a.th, o,
-                                   sp.density, csf, pdiv);
+      reg_phase<T, R, 0, PRESSURE, ONE_SYNC>(t, sh, vt0, vt1, lane, w, rbase, s, nphase, K, th, o,
+                                             sp.density, csf, pdiv);
+      reg_phase<T, R, 1, PRESSURE, ONE_SYNC>(t, sh, vt0, vt1, lane, w, rbase, s + 1, nphase, K,
+                                             th, o, sp.density, csf, pdiv);
     }
   } else {
     for (int s = 0; s < nphase; s += 2) {
-      reg_phase<T, R, 1, PRESSURE, ONE_SYNC>(t, sh, vt0, vt1, lane, w, rbase, s, nphase, K, a.th, o,
-                                   sp.density, csf, pdiv);
-      reg_phase<T, R, 0, PRESSURE, ONE_SYNC>(t, sh, vt0, vt1, lane, w, rbase, s + 1, nphase, K, a.th, o,
-                                   sp.density, csf, pdiv);
+      reg_phase<T, R, 1, PRESSURE, ONE_SYNC>(t, sh, vt0, vt1, lane, w, rbase, s, nphase, K, th, o,
+                                             sp.density, csf, pdiv);
+      reg_phase<T, R, 0, PRESSURE, ONE_SYNC>(t, sh, vt0, vt1, lane, w, rbase, s + 1, nphase, K,
+                                             th, o, sp.density, csf, pdiv);
     }
   }
+}
 
-  // write back the owned tile (lane and column recomputed from a fresh
-  // thread index, so that nothing of the load is kept live over the sweeps)
-  int tid_s = (int)threadIdx.x;
-  asm volatile("" : "+v"(tid_s));
-  const int lane_s = tid_s & 63;
-  const int gi_s = gx0 + 2 * lane_s;
-  const bool own0 = lc_in_tile(2 * lane_s, K, a.tw) && (unsigned)gi_s < (unsigned)W;
-  const bool own1 = lc_in_tile(2 * lane_s + 1, K, a.tw) && (unsigned)(gi_s + 1) < (unsigned)W;
+// Writes back the owned tile (RING: only its cells within 2K of the tile's
+// edge, which are the halos of the neighbouring tiles, and no pressure).
+template <typename T, bool PRESSURE, int R, bool RING>
+__device__ __attribute__((always_inline)) inline void reg_store(const RegTile<R>& t,
+                                                                T* __restrict__ u_out,
+                                                                T* __restrict__ v_out,
+                                                                float* __restrict__ p,
+                                                                const StepParams& sp,
+                                                                const RegArgs& a,
+                                                                const RegPlace& pl) {
+  const int lane = reg_lane();
+  const int K = pl.K, W = sp.W, H = sp.H, P = sp.P;
+  const int gi = pl.gx0 + 2 * lane;
+  bool own0 = lc_in_tile(2 * lane, K, a.tw) && (unsigned)gi < (unsigned)W;
+  bool own1 = lc_in_tile(2 * lane + 1, K, a.tw) && (unsigned)(gi + 1) < (unsigned)W;
+  // RING: columns within 2K of the tile's left or right edge (pairs never
+  // straddle that bound: 2K and the tile width are even)
+  const int oc = 2 * lane - 2 * K;
+  const bool ring_col = oc < 2 * K || oc >= a.tw - 2 * K;
 #pragma unroll
   for (int rho = 0; rho < R; rho++) {
-    const int r = rbase + rho;
-    const int gj = gy0 + r;
+    const int r = pl.rbase + rho;
+    const int gj = pl.gy0 + r;
     if (r < 2 * K || r >= 2 * K + a.th || gj >= H) continue;
-    const long gk = (long)gj * P + gi_s;
-    store_pair(u_out, gk, own0, own1, t.u0[rho], t.u1[rho]);
-    store_pair(v_out, gk, own0, own1, t.v0[rho], t.v1[rho]);
-    if constexpr (PRESSURE) {
-      if (own0) p[gk] = t.s0[rho];
-      if (own1) p[gk + 1] = t.s1[rho];
+    bool s0 = own0, s1 = own1;
+    if constexpr (RING) {
+      const int orow = r - 2 * K;
+      const bool ring_row = orow < 2 * K || orow >= a.th - 2 * K;
+      s0 = s0 && (ring_row || ring_col);
+      s1 = s1 && (ring_row || ring_col);
+    }
+    const long gk = (long)gj * P + gi;
+    if constexpr (RING) {
+      coherent_store_pair(u_out, gk, s0, s1, t.u0[rho], t.u1[rho]);
+      coherent_store_pair(v_out, gk, s0, s1, t.v0[rho], t.v1[rho]);
+    } else {
+      store_pair(u_out, gk, s0, s1, t.u0[rho], t.u1[rho]);
+      store_pair(v_out, gk, s0, s1, t.v0[rho], t.v1[rho]);
+    }
+    if constexpr (PRESSURE && !RING) {
+      if (s0) p[gk] = t.s0[rho];
+      if (s1) p[gk + 1] = t.s1[rho];
     }
   }
+}
+
+// Reloads u and v of the halo (every register cell outside the owned tile)
+// from the neighbours' stored rings; the owned cells keep their registers.
+// Loads first (masked to the halo lanes), one wait, then cells outside the
+// grid are set to 0, as at the first load.
+template <typename T, int R>
+__device__ __attribute__((always_inline)) inline void reg_reload_halo(RegTile<R>& t,
+                                                                      const T* __restrict__ u_in,
+                                                                      const T* __restrict__ v_in,
+                                                                      const StepParams& sp,
+                                                                      const RegArgs& a,
+                                                                      const RegPlace& pl) {
+  const int lane = reg_lane();
+  const int K = pl.K, W = sp.W, H = sp.H, P = sp.P;
+  const int gi = pl.gx0 + 2 * lane;
+  const bool col0 = (unsigned)gi < (unsigned)W, col1 = (unsigned)(gi + 1) < (unsigned)W;
+  const bool halo_lane = lane < K || lane >= 64 - K;
+  const int gic = min(max(gi, 0), P - 2);
+  float nu0[R], nu1[R], nv0[R], nv1[R];
+#pragma unroll
+  for (int rho = 0; rho < R; rho++) {
+    const int r = pl.rbase + rho;
+    const bool halo_row = r < 2 * K || r >= 2 * K + a.th;
+    nu0[rho] = t.u0[rho];
+    nu1[rho] = t.u1[rho];
+    nv0[rho] = t.v0[rho];
+    nv1[rho] = t.v1[rho];
+    if (halo_row || halo_lane) {
+      const long k = (long)min(max(pl.gy0 + r, 0), H - 1) * P + gic;
+      coherent_load_pair(u_in, k, nu0[rho], nu1[rho]);
+      coherent_load_pair(v_in, k, nv0[rho], nv1[rho]);
+    }
+  }
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int rho = 0; rho < R; rho++) {
+    const int gj = pl.gy0 + pl.rbase + rho;
+    const bool row_in = (unsigned)gj < (unsigned)H;
+    const bool in0 = row_in && col0, in1 = row_in && col1;
+    t.u0[rho] = in0 ? nu0[rho] : 0.0f;
+    t.v0[rho] = in0 ? nv0[rho] : 0.0f;
+    t.u1[rho] = in1 ? nu1[rho] : 0.0f;
+    t.v1[rho] = in1 ? nv1[rho] : 0.0f;
+  }
+}
+
+// One launch of the register-resident projection for tile `tile` (row-major
+// over the grid's ntx tiles per row), run by a workgroup of 64 * kRegWaves
+// threads.
+template <typename T, bool PRESSURE, int R, bool ONE_SYNC>
+__device__ __attribute__((always_inline)) inline void project_regs_tile(
+    const StepParams& sp, const uint8_t* __restrict__ gates, const T* __restrict__ u_in,
+    const T* __restrict__ v_in, T* __restrict__ u_out, T* __restrict__ v_out,
+    float* __restrict__ p, const RegArgs& a, int tile, RegShared& sh) {
+  const RegPlace pl = reg_place<R>(a, tile);
+  RegTile<R> t;
+  reg_load<T, PRESSURE, R>(t, sp, gates, u_in, v_in, p, a, pl);
+  reg_sweeps<T, PRESSURE, R, ONE_SYNC>(t, sh, sp, pl, a.th, a.sweeps);
+  reg_store<T, PRESSURE, R, false>(t, u_out, v_out, p, sp, a, pl);
 }
 
 }  // namespace osy

@@ -296,7 +296,7 @@ def test_autotune_keeps_state_and_results():
     fl, o = pair(cfg, seed=14)
     before = {n: fl.get_field(n) for n, _ in FIELDS}
     tuning, ms = fl.autotune()
-    assert ms > 0 and tuning["projection_kernel"] in (0, 1, 2)
+    assert ms > 0 and tuning["projection_kernel"] in (0, 1, 2, 3)
     for n, _ in FIELDS:
         assert np.array_equal(fl.get_field(n), before[n])
     fl.step(3)
