@@ -31,11 +31,13 @@ HBM_PEAK_GBS = 8000.0
 
 # HBM bytes per projection launch measured with rocprofv3 FETCH_SIZE and
 # WRITE_SIZE passes (scaled by the same counters on a copy of known size),
-# per (kernel, tile width, tile height, sweeps per launch, W, H); summary and
-# commands in profiles/step_1920x1080_n50_fp32.md.
+# per (kernel, tile width, tile height, sweeps per launch, W, H); summaries
+# and commands in profiles/step_1920x1080_n50_fp32.md and (kernel 3, one
+# launch of all n = 50 sweeps) profiles/resident_1920x1080_n50_fp32.md.
 MEASURED_LAUNCH_TRAFFIC = {
     (2, 108, 44, 5, 1920, 1080): 38_090_423,
     (2, 104, 40, 6, 1920, 1080): 38_554_377,
+    (3, 108, 44, 5, 1920, 1080): 248_574_510,
 }
 METRIC = "Gcell-steps/s at 1920x1080 n=50; % HBM roofline; p50/p99 batched step latency"
 

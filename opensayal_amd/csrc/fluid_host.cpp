@@ -503,9 +503,15 @@ int osy_fluid_set_tuning(osy_fluid* fl, const osy_tuning* t) {
     pt.kernel = PROJ_COLOUR;
   } else if (pt.kernel == PROJ_RESIDENT && pt.valid(fl->cfg.enable_pressure != 0) &&
              !tuning_fits(fl, pt)) {
+    int slots = 0;
+    const hipError_t e = resident_capacity(fl->storage, fl->cfg.enable_pressure != 0,
+                                           pt.reg_rows(), &slots);
     return set_error(OSY_E_INVALID,
                      "resident projection: " + std::to_string(pt.tiles(fl->W, fl->H)) +
-                         " tiles do not all fit on the device at once");
+                         " tiles do not all fit on the device at once (" +
+                         std::to_string(slots) + " workgroup slots at " +
+                         std::to_string(pt.reg_rows()) + " rows per wavefront; " +
+                         hipGetErrorString(e) + ")");
   } else if (!pt.valid(fl->cfg.enable_pressure != 0)) {
     return set_error(OSY_E_INVALID,
                      "tiled projection needs 1 <= k <= 64, an even tile width >= 2, a tile "

@@ -475,19 +475,6 @@ hipError_t allow_large_lds() {
   return e;
 }
 
-template <typename T>
-hipError_t resident_capacity_t(bool pressure, int rows, int* slots) {
-  int dev = 0, cus = 0, per_cu = 0;
-  hipError_t e = hipGetDevice(&dev);
-  if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-  if (e == hipSuccess)
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        &per_cu,
-        pressure ? resident_kernel<T, true>(rows) : resident_kernel<T, false>(rows),
-        64 * kRegWaves, 0);
-  *slots = e == hipSuccess ? per_cu * cus : 0;
-  return e;
-}
 
 }  // namespace
 
@@ -498,12 +485,21 @@ hipError_t init_projection_kernels() {
   return e;
 }
 
+// Workgroups of kRegWaves wavefronts a CU holds at once, from the register
+// kernels' launch bounds (which the compiler meets): at most 80 VGPRs at 4
+// rows (6 wavefronts per SIMD: 3 workgroups), 128 at 8 and 10 rows (4: 2
+// workgroups), up to 256 otherwise (1).  LDS (12 KiB per workgroup) and the
+// wavefront slots allow more.  (The runtime's occupancy query was not used:
+// it has returned 0 for a kernel that fits, depending on what ran before.)
 hipError_t resident_capacity(int storage, bool pressure, int rows, int* slots) {
-  switch (storage) {
-    case OSY_STORAGE_F16: return resident_capacity_t<__half>(pressure, rows, slots);
-    case OSY_STORAGE_BF16: return resident_capacity_t<__hip_bfloat16>(pressure, rows, slots);
-    default: return resident_capacity_t<float>(pressure, rows, slots);
-  }
+  (void)storage;
+  (void)pressure;
+  int dev = 0, cus = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  const int per_cu = rows == 4 ? 3 : (rows == 8 || rows == 10) ? 2 : 1;
+  *slots = e == hipSuccess ? per_cu * cus : 0;
+  return e;
 }
 
 template <typename T>
