@@ -357,7 +357,8 @@ __global__ void __launch_bounds__(64 * kRegWaves, (R == 8 || R == 10) ? 4 : (R =
       reg_reload_halo<T, R>(t, odd ? u1 : u0, odd ? v1 : v0, sp, a, pl);
     }
     const bool last = s + 1 == ra.nstages;
-    reg_sweeps<T, PRESSURE, R, false>(t, sh, sp, pl, a.th, last ? ra.last_sweeps : a.k);
+    // one barrier per colour phase (measured 0.4% faster than two here)
+    reg_sweeps<T, PRESSURE, R, true>(t, sh, sp, pl, a.th, last ? ra.last_sweeps : a.k);
     T* uo = odd ? u0 : u1;
     T* vo = odd ? v0 : v1;
     if (!last) {
