@@ -260,15 +260,14 @@ __device__ inline float mask_face(float d, uint32_t m) {
 // s = 0 moves no face, so its quotient is never used, and almost every
 // other cell has s = 4: a row whose lanes all have s = 0 or 4 takes
 // div * 0.25.  The rest (`mixed`, wavefront uniform) take div * 2^-log2(s)
-// for s = 1, 2, 4 (log2(s) = 8s >> 4), exact, and the IEEE quotient for
-// s = 3.
+// for s = 1, 2, 4 (log2(s) = 8s >> 4), exact, and third_quotient for s = 3
+// (the IEEE quotient div / 3 for every binary32 div).
 __device__ inline float mask_quotient(float div, uint32_t m, bool mixed) {
   float q = div * 0.25f;
   if (mixed) {
     const uint32_t c8 = (uint32_t)__builtin_popcount(m);
     const float qp = __builtin_amdgcn_ldexpf(div, -(int)(c8 >> 4));
-    const float qs = div / (float)(c8 >> 3);
-    q = c8 == 24u ? qs : qp;
+    q = c8 == 24u ? third_quotient(div) : qp;
   }
   return q;
 }
