@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define OSY_ABI_VERSION 6
+#define OSY_ABI_VERSION 7
 
 typedef enum osy_status {
   OSY_OK = 0,
@@ -292,6 +292,16 @@ int osy_fluid_traces_noalloc(osy_fluid* fluid, int32_t distance, int32_t length,
  * fit the kernel's limits).  Captured step graphs are rebuilt on next use. */
 int osy_fluid_set_tuning(osy_fluid* fluid, const osy_tuning* tuning);
 int osy_fluid_get_tuning(const osy_fluid* fluid, osy_tuning* tuning);
+
+/* Bound of one neighbour wait of the resident projection (projection_kernel
+ * 3), in microseconds of the device's wall clock (default 2,000,000).  A
+ * wait that exceeds it sets the handle's error word and the launch drains
+ * without waiting; the next step, synchronize or field read then returns
+ * OSY_E_STATE.  No counterpart in the reference: for tests of that path (a
+ * limit of 0) and for callers that want a stalled step to fail sooner.
+ * OSY_E_INVALID for a negative limit.  Captured step graphs are rebuilt on
+ * next use. */
+int osy_fluid_set_wait_limit(osy_fluid* fluid, int64_t microseconds);
 
 /* Autotuner (this implementation's own; no counterpart in the reference):
  * times every supported projection launch shape on this grid, projecting a

@@ -165,6 +165,11 @@ class Fluid:
         check(lib().osy_fluid_set_tuning(self._h, ctypes.byref(t)))
         self._refresh_info()
 
+    def set_wait_limit(self, microseconds: int) -> None:
+        """Bound of one neighbour wait of the resident projection (default
+        2 s); a step whose wait exceeds it raises OSY_E_STATE."""
+        check(lib().osy_fluid_set_wait_limit(self._h, int(microseconds)))
+
     def autotune(self) -> tuple:
         """Time every projection launch shape on this grid and keep the
         fastest; returns (tuning dict, ms per projection)."""

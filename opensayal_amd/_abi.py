@@ -137,6 +137,7 @@ PROTOTYPES = {
                                                 ctypes.c_void_p, ctypes.c_size_t]),
     "osy_fluid_set_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(OsyTuning)]),
     "osy_fluid_get_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(OsyTuning)]),
+    "osy_fluid_set_wait_limit": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64]),
     "osy_fluid_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(OsyTuning),
                                           ctypes.POINTER(ctypes.c_double)]),
     "osy_fluid_autotune_cached": (ctypes.c_int, [ctypes.c_void_p]),

@@ -141,7 +141,8 @@ int create_impl(const osy_config* cfg, osy_fluid** out) {
     delete fl;
     return set_error(OSY_E_HIP, std::string("device attribute: ") + hipGetErrorString(e));
   }
-  fl->spin_ticks = (long long)(khz > 0 ? khz : 100000) * 2000;
+  fl->wall_khz = khz > 0 ? khz : 100000;
+  fl->spin_ticks = (long long)fl->wall_khz * 2000;  // 2 s
   fl->block_bytes = L.total();
   e = hipMalloc(&fl->block, fl->block_bytes);
   if (e != hipSuccess) {
@@ -529,6 +530,20 @@ int osy_fluid_set_tuning(osy_fluid* fl, const osy_tuning* t) {
         g.exec = nullptr;
       }
   fl->tuning = pt;
+  return OSY_OK;
+}
+
+int osy_fluid_set_wait_limit(osy_fluid* fl, int64_t microseconds) {
+  if (!fl) return set_error(OSY_E_INVALID, "null handle");
+  if (microseconds < 0) return set_error(OSY_E_INVALID, "the wait limit must be >= 0");
+  OSY_HIP_CHECK(hipStreamSynchronize(fl->stream));
+  for (auto& row : fl->graphs)
+    for (auto& g : row)
+      if (g.exec) {
+        OSY_HIP_CHECK(hipGraphExecDestroy(g.exec));
+        g.exec = nullptr;
+      }
+  fl->spin_ticks = (long long)(microseconds * (int64_t)fl->wall_khz / 1000);
   return OSY_OK;
 }
 

@@ -35,7 +35,8 @@ struct osy_fluid {
   void* trace_buf = nullptr;  // path-line output, grown on demand
   size_t trace_cap = 0;
   ProjTuning tuning;  // projection kernel shape, chosen at create
-  long long spin_ticks = 0;  // 2 s of the device's wall clock
+  long long spin_ticks = 0;  // bound of a resident projection's neighbour wait
+  int wall_khz = 100000;     // the device's wall-clock rate
   osy_batch* batch = nullptr;  // the batch stepping this handle, if any
   struct Graph {
     hipGraphExec_t exec = nullptr;

@@ -29,7 +29,7 @@ def test_library_exports_every_declared_function():
     for name in names:
         assert hasattr(lib, name), name
         assert name in _abi.PROTOTYPES, name
-    assert lib.osy_abi_version() == 6
+    assert lib.osy_abi_version() == 7
 
 
 def test_struct_sizes_match_header_layout():

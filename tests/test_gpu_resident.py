@@ -134,3 +134,25 @@ def test_resident_refuses_shapes_that_do_not_fit():
     with pytest.raises(OpenSayalError, match="fit"):
         fl.set_tuning(projection_kernel=3, tile_width=124, tile_height=28, sweeps_per_launch=1)
     fl.set_tuning(projection_kernel=2, tile_width=124, tile_height=28, sweeps_per_launch=1)
+
+
+def test_resident_wait_timeout_is_reported_and_recovers():
+    """A neighbour wait bounded at 0 times out: the launch drains without
+    waiting, the step reports OSY_E_STATE, and with the bound restored the
+    handle steps exactly again (the flags were cleared, the error word
+    read and cleared)."""
+    cfg = wl.plain_full()
+    fl, o = pair(cfg, seed=28)
+    fl.set_tuning(**RESIDENT[0])
+    fl.set_wait_limit(0)
+    with pytest.raises(OpenSayalError, match="timed out"):
+        fl.step(1)
+    fl.set_wait_limit(2_000_000)
+    u, v, s = wl.initial_state(cfg, 28)
+    for name, arr in (("vel_x", u), ("vel_y", v), ("smoke", s)):
+        fl.set_field(name, arr)
+    fl.step(2)
+    o.update(steps=2)
+    assert_bitwise(fl, o, "after a timed-out launch")
+    with pytest.raises(OpenSayalError):
+        fl.set_wait_limit(-1)
