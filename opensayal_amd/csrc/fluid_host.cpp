@@ -257,9 +257,7 @@ bool tuning_fits(const osy_fluid* fl, const ProjTuning& t) {
   if (t.kernel != PROJ_RESIDENT) return true;
   const int tiles = t.tiles(fl->W, fl->H);
   int slots = 0;
-  if (resident_capacity(fl->storage, fl->cfg.enable_pressure != 0, t.reg_rows(), &slots) !=
-      hipSuccess)
-    return false;
+  if (resident_capacity(t.reg_rows(), &slots) != hipSuccess) return false;
   return tiles <= kResidentMaxTiles && tiles <= slots;
 }
 
@@ -505,8 +503,7 @@ int osy_fluid_set_tuning(osy_fluid* fl, const osy_tuning* t) {
   } else if (pt.kernel == PROJ_RESIDENT && pt.valid(fl->cfg.enable_pressure != 0) &&
              !tuning_fits(fl, pt)) {
     int slots = 0;
-    const hipError_t e = resident_capacity(fl->storage, fl->cfg.enable_pressure != 0,
-                                           pt.reg_rows(), &slots);
+    const hipError_t e = resident_capacity(pt.reg_rows(), &slots);
     return set_error(OSY_E_INVALID,
                      "resident projection: " + std::to_string(pt.tiles(fl->W, fl->H)) +
                          " tiles do not all fit on the device at once (" +

@@ -492,9 +492,7 @@ hipError_t init_projection_kernels() {
 // workgroups), up to 256 otherwise (1).  LDS (12 KiB per workgroup) and the
 // wavefront slots allow more.  (The runtime's occupancy query was not used:
 // it has returned 0 for a kernel that fits, depending on what ran before.)
-hipError_t resident_capacity(int storage, bool pressure, int rows, int* slots) {
-  (void)storage;
-  (void)pressure;
+hipError_t resident_capacity(int rows, int* slots) {
   int dev = 0, cus = 0;
   hipError_t e = hipGetDevice(&dev);
   if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);

@@ -13,7 +13,7 @@ hipError_t init_projection_kernels();
 // Workgroups of the resident projection (PROJ_RESIDENT) with `rows` rows per
 // wavefront that the current device holds at once: a shape fits when its
 // tile count is at most this.
-hipError_t resident_capacity(int storage, bool pressure, int rows, int* slots);
+hipError_t resident_capacity(int rows, int* slots);
 
 // Enqueues `sp.n` sweeps of the projection.  `*cur` is the u/v buffer index
 // holding the input; on return it names the buffer holding the result.

@@ -106,6 +106,18 @@ def test_resident_half_storage_bit_exact(storage):
     assert_bitwise(fl, o, f"{storage} resident")
 
 
+@pytest.mark.parametrize("storage", [0, 1, 2])
+def test_resident_odd_width_bit_exact(storage):
+    """An odd width: the last column's cells are stored (and their rings
+    exchanged) one element at a time, in every storage type."""
+    cfg = obstacle_grid(251, 131, projection_n=12, storage=storage)
+    fl, o = pair(cfg, seed=29)
+    fl.set_tuning(**RESIDENT[3])
+    fl.step(3)
+    o.update(steps=3)
+    assert_bitwise(fl, o, f"251x131 storage {storage}")
+
+
 def test_resident_agrees_with_launch_per_stage():
     """The same shape as one resident launch and as launches of k sweeps
     give the same bits over many steps (no oracle: 40 steps at 640x360)."""
