@@ -38,6 +38,9 @@ MEASURED_LAUNCH_TRAFFIC = {
     (2, 108, 44, 5, 1920, 1080): 38_090_423,
     (2, 104, 40, 6, 1920, 1080): 38_554_377,
     (3, 108, 44, 5, 1920, 1080): 248_500_478,
+    (3, 96, 48, 8, 1920, 1080): 256_970_333,
+    (3, 104, 56, 6, 1920, 1080): 263_479_390,
+    (3, 112, 48, 4, 1920, 1080): 256_964_802,
 }
 METRIC = "Gcell-steps/s at 1920x1080 n=50; % HBM roofline; p50/p99 batched step latency"
 
