@@ -50,7 +50,7 @@ def assert_bitwise(fl, o, what):
 
 
 def test_full_size_1000_steps_bit_exact():
-    """BASELINE plain-full: 1920x1080, n = 50, o = 1.9, 1000 steps."""
+    """The plain wind-tunnel workload at full size: 1920x1080, n = 50, o = 1.9, 1000 steps."""
     cfg = wl.plain_full()
     fl, o = pair(cfg, *wl.initial_state(cfg, 3))
     fl.step(100)
