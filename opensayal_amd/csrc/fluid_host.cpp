@@ -193,6 +193,7 @@ int create_impl(const osy_config* cfg, osy_fluid** out) {
 }
 
 int enqueue_eager(osy_fluid* fl, const osy_source* src, float d_t, hipStream_t s) {
+  if (fl->tuning.kernel == PROJ_RESIDENT) fl->resident_pending = true;
   StepParams sp = fl->params(d_t, src);
   enqueue_step(fl->storage, sp, fl->f, &fl->uv, &fl->sm, fl->tuning, s);
   OSY_HIP_CHECK(hipGetLastError());
@@ -200,6 +201,7 @@ int enqueue_eager(osy_fluid* fl, const osy_source* src, float d_t, hipStream_t s
 }
 
 int enqueue_graph_steps(osy_fluid* fl, int32_t steps, float d_t, hipStream_t s) {
+  if (steps > 0 && fl->tuning.kernel == PROJ_RESIDENT) fl->resident_pending = true;
   for (int32_t k = 0; k < steps; k++) {
     osy_fluid::Graph& g = fl->graphs[fl->uv][fl->sm];
     if (!g.exec || g.d_t != d_t) {
@@ -262,7 +264,8 @@ bool tuning_fits(const osy_fluid* fl, const ProjTuning& t) {
 }
 
 int fluid_check_error(osy_fluid* fl) {
-  if (fl->tuning.kernel != PROJ_RESIDENT) return OSY_OK;
+  if (!fl->resident_pending) return OSY_OK;
+  fl->resident_pending = false;
   int err = 0;
   OSY_HIP_CHECK(hipMemcpy(&err, fl->f.error, sizeof(int), hipMemcpyDeviceToHost));
   if (err) {
@@ -371,6 +374,7 @@ int osy_fluid_project(osy_fluid* fl, int32_t sweeps, void* hip_stream) {
   hipStream_t s = hip_stream ? (hipStream_t)hip_stream : fl->stream;
   StepParams sp = fl->params(fl->cfg.d_t, nullptr);
   sp.n = sweeps;
+  if (fl->tuning.kernel == PROJ_RESIDENT) fl->resident_pending = true;
   enqueue_projection_only(fl->storage, sp, fl->f, &fl->uv, fl->tuning, s);
   OSY_HIP_CHECK(hipGetLastError());
   return OSY_OK;
@@ -630,6 +634,8 @@ int osy_fluid_autotune(osy_fluid* fl, osy_tuning* chosen, double* ms) {
   ProjTuning best = fl->tuning;
   if (sp.n > 0) {
     OSY_HIP_CHECK(hipStreamSynchronize(fl->stream));
+    // a timed-out step before the trials is reported, not cleared by them
+    if (int st = fluid_check_error(fl)) return st;
     // snapshot of the current u, v and pressure; every trial starts from it
     const size_t fld = (size_t)fl->P * fl->H * fl->elem;
     const size_t pb = (size_t)fl->P * fl->H * 4;
@@ -683,6 +689,11 @@ int osy_fluid_autotune(osy_fluid* fl, osy_tuning* chosen, double* ms) {
     hipError_t r = restore(fl->stream);
     if (e == hipSuccess) e = r;
     r = hipStreamSynchronize(fl->stream);
+    if (e == hipSuccess) e = r;
+    // a resident trial whose wait timed out only mistimed itself: the state
+    // was restored from the snapshot, so its error word is cleared
+    const int zero = 0;
+    r = hipMemcpy(fl->f.error, &zero, sizeof(int), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = r;
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);

@@ -37,6 +37,8 @@ struct osy_fluid {
   ProjTuning tuning;  // projection kernel shape, chosen at create
   long long spin_ticks = 0;  // bound of a resident projection's neighbour wait
   int wall_khz = 100000;     // the device's wall-clock rate
+  bool resident_pending = false;  // a resident projection ran since the last
+                                  // read of the error word
   osy_batch* batch = nullptr;  // the batch stepping this handle, if any
   struct Graph {
     hipGraphExec_t exec = nullptr;
@@ -114,7 +116,8 @@ size_t fluid_block_bytes(int W, int H, int storage);
 bool tuning_fits(const osy_fluid* fl, const ProjTuning& t);
 
 // Reads and clears the device error word (a resident projection whose
-// neighbour wait timed out); call with the stream synchronized.
+// neighbour wait timed out) when a resident projection ran since the last
+// read; call with the stream synchronized.
 int fluid_check_error(osy_fluid* fl);
 
 // Batch membership hooks (batch_host.cpp): wait for the batch's queued

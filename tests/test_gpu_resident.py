@@ -168,3 +168,11 @@ def test_resident_wait_timeout_is_reported_and_recovers():
     assert_bitwise(fl, o, "after a timed-out launch")
     with pytest.raises(OpenSayalError):
         fl.set_wait_limit(-1)
+    # a timeout of an asynchronous step is reported by the next synchronize
+    # even when the launch shape changed in between
+    fl.set_wait_limit(0)
+    fl.step_async(1)
+    fl.set_tuning(**dict(RESIDENT[0], projection_kernel=2))
+    with pytest.raises(OpenSayalError, match="timed out"):
+        fl.synchronize()
+    fl.synchronize()
