@@ -76,6 +76,20 @@ def test_full_size_1000_steps_bit_exact():
     assert g[1] == r[1]
 
 
+@pytest.mark.parametrize("tile_width,tile_height,k", [(108, 44, 5), (96, 48, 8)])
+def test_full_size_1000_steps_resident_bit_exact(tile_width, tile_height, k):
+    """The same 1000 steps with the resident projection (all 50 sweeps of a
+    step in one launch), in the shapes the autotuner picks at this size."""
+    cfg = wl.plain_full()
+    fl, o = pair(cfg, *wl.initial_state(cfg, 4))
+    fl.set_tuning(projection_kernel=3, tile_width=tile_width, tile_height=tile_height,
+                  sweeps_per_launch=k)
+    for done in (500, 1000):
+        fl.step(500)
+        o.update(steps=500)
+        assert_bitwise(fl, o, f"resident {tile_width}x{tile_height} k={k}, {done} steps")
+
+
 def test_full_size_pressure_on_bit_exact():
     """The same grid with the pressure solve on (pressure sums in every
     sweep, the forces zeroing them): 20 steps bit-exact, pressure field and
