@@ -299,8 +299,8 @@ void launch_tiled(const StepParams& sp, const GridView& g, const FieldPtrs& f, i
 // The last workgroup to finish clears the flags for the next launch.
 //
 // Every tile's workgroup must be resident at once (the host checks the
-// tile count against the occupancy); a wait is bounded by spin_ticks, after
-// which the error word is set and the launch drains without waiting.
+// tile count against resident_capacity); a wait is bounded by spin_ticks,
+// after which the error word is set and the launch drains without waiting.
 struct ResidentArgs {
   int nstages;      // ceil(n / k)
   int last_sweeps;  // sweeps of the last stage
