@@ -349,7 +349,9 @@ int osy_batch_set_tuning(osy_batch* batch, const osy_batch_tuning* tuning);
 int osy_batch_get_tuning(const osy_batch* batch, osy_batch_tuning* tuning);
 /* Autotuner: times each supported shape on the current members (their
  * state is saved first and restored afterwards), keeps the fastest; writes
- * it to `chosen` and its time per batch step to `ms` (either may be NULL). */
+ * it to `chosen` and its time per batch step to `ms` (either may be NULL).
+ * Members that step on their own graphs (osy_batch_set_per_grid_cells) get
+ * their projection shape from osy_fluid_autotune first. */
 int osy_batch_autotune(osy_batch* batch, osy_batch_tuning* chosen, double* ms);
 int osy_batch_get_info(const osy_batch* batch, osy_batch_info* info);
 /* Members of at least `cells` cells (W*H) fill the GPU on their own: each

@@ -477,6 +477,14 @@ int osy_batch_autotune(osy_batch* b, osy_batch_tuning* chosen, double* ms) {
   int st = sync_batch(b);
   if (st != OSY_OK) return st;
   if (b->members.empty()) return set_error(OSY_E_STATE, "the batch has no members");
+  // members that step on their own graphs: the per-grid autotuner picks
+  // their projection shape (their graphs run one after another on the
+  // batch's stream, so a resident projection is safe there)
+  for (osy_fluid* fl : b->members)
+    if (b->per_grid_cells > 0 && (int64_t)fl->W * fl->H >= b->per_grid_cells) {
+      st = fluid_autotune(fl, nullptr, nullptr);
+      if (st != OSY_OK) return st;
+    }
   // save every member's device state and buffer parity
   size_t total = 0;
   for (osy_fluid* fl : b->members) total += fl->block_bytes;

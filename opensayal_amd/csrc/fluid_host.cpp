@@ -627,6 +627,12 @@ extern "C" {
 int osy_fluid_autotune(osy_fluid* fl, osy_tuning* chosen, double* ms) {
   if (!fl) return set_error(OSY_E_INVALID, "null handle");
   if (fl->batch) return set_error(OSY_E_STATE, "the handle is stepped by a batch");
+  return fluid_autotune(fl, chosen, ms);
+}
+
+}  // extern "C"
+
+int osy::fluid_autotune(osy_fluid* fl, osy_tuning* chosen, double* ms) {
   const bool pressure = fl->cfg.enable_pressure != 0;
   StepParams sp = fl->params(fl->cfg.d_t, nullptr);
   sp.forces_in_projection = 0;
@@ -709,6 +715,8 @@ int osy_fluid_autotune(osy_fluid* fl, osy_tuning* chosen, double* ms) {
   if (ms) *ms = best_ms;
   return OSY_OK;
 }
+
+extern "C" {
 
 int osy_fluid_autotune_cached(osy_fluid* fl) {
   if (!fl) return set_error(OSY_E_INVALID, "null handle");

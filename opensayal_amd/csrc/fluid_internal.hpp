@@ -115,6 +115,10 @@ size_t fluid_block_bytes(int W, int H, int storage);
 // once on its device.
 bool tuning_fits(const osy_fluid* fl, const ProjTuning& t);
 
+// osy_fluid_autotune without the batch-membership check: the batch
+// autotuner tunes the projection of members that step on their own graphs.
+int fluid_autotune(osy_fluid* fl, osy_tuning* chosen, double* ms);
+
 // Reads and clears the device error word (a resident projection whose
 // neighbour wait timed out) when a resident projection ran since the last
 // read; call with the stream synchronized.

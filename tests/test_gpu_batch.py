@@ -331,3 +331,31 @@ def test_large_members_step_on_their_own_graphs():
             fl.close()
     with Batch() as b, pytest.raises(OpenSayalError):
         b.set_per_grid_cells(-1)
+
+
+def test_batch_autotune_tunes_large_members():
+    """The batch autotuner also picks the projection shape of the members
+    that step on their own graphs (the per-grid autotuner's choice, which a
+    fresh handle of the same shape then adopts from its cache), and the
+    fields still equal the runs alone."""
+    big = make_config(1100, 620, projection_n=23, wind_speed=2.0, g=-5.0, pipe_height=150,
+                      obstacle_center_x=400, obstacle_center_y=300, obstacle_radius=30.0)
+    cfgs = [big] + mixed_configs()[:2]
+    batched, alone = pairs(cfgs, seed0=310)
+    with Batch() as b:
+        b.set_per_grid_cells((1 << 20) // 2)
+        for fl in batched:
+            b.add(fl)
+        b.autotune()
+        member = batched[0]
+        other = Fluid(big)
+        assert other.autotune_cached()
+        assert member.tuning == other.tuning
+        other.close()
+        b.step(3)
+        for fl in alone:
+            fl.step(3)
+        for i, (x, y) in enumerate(zip(batched, alone)):
+            assert_same(state(x), state(y), f"member {i}")
+    for fl in batched + alone:
+        fl.close()
