@@ -317,6 +317,13 @@ __device__ inline float inv_distance(float dx, float dy) {
   return inv_from_distance(sqrtf(dx * dx + dy * dy));
 }
 
+// inv_distance for a sample inside the staged region: its distances to the
+// four cell centres are a few cells at most, so the distance is finite and
+// the infinite case of inv_from_distance never arises.
+__device__ inline float inv_distance_staged(float dx, float dy) {
+  return (float)recip_nr((double)sqrtf(dx * dx + dy * dy) + 1e-6);
+}
+
 // inv / sum rounded to float (the weights of src/fluid.cu:700-712), given
 // rs = recip_nr(sum): (float)(inv * rs) in double.  Exact: rs is within
 // 2^-50 of 1 / sum and the product rounds once more (2^-53), so the double
@@ -483,10 +490,10 @@ __device__ inline float lds_sample_smoke(int cs, float x, float y, const float* 
   const int j3 = low ? j - 1 : j + 1;
   const float x1 = centre<CS1>(i, cs), y1 = centre<CS1>(j, cs);
   const float x2 = centre<CS1>(i2, cs), y3 = centre<CS1>(j3, cs);
-  const float inv1 = inv_distance(x - x1, y - y1);
-  const float inv2 = inv_distance(x - x2, y - y1);
-  const float inv3 = inv_distance(x - x1, y - y3);
-  const float inv4 = inv_distance(x - x2, y - y3);
+  const float inv1 = inv_distance_staged(x - x1, y - y1);
+  const float inv2 = inv_distance_staged(x - x2, y - y1);
+  const float inv3 = inv_distance_staged(x - x1, y - y3);
+  const float inv4 = inv_distance_staged(x - x2, y - y3);
   const float sum_inv = ((inv1 + inv2) + inv3) + inv4;
   const double rs = recip_nr((double)sum_inv);
   const int b1 = (j - y0) * RX + (i - x0);
