@@ -407,7 +407,10 @@ int osy_calibration_copy(uint64_t bytes, int32_t reps);
  *      packed conversion) against one scalar conversion per value;
  *   5, 6: the pressure sums' quotient x / d_t as a double multiply by the
  *      double nearest 1 / d_t, against IEEE division, for d_t = 0.05 and
- *      d_t = 1/60.
+ *      d_t = 1/60;
+ *   7: the staged smoke weights' square root (the hardware root corrected
+ *      from fused residuals) against the correctly rounded sqrtf, on 0 and
+ *      every finite value from 2^-96 up.
  * Writes the number of inputs that fail and the smallest failing pattern
  * (0xFFFFFFFF when none); synchronous. */
 int osy_arith_check(int32_t mode, uint64_t* mismatches, uint32_t* first_input);

@@ -1,7 +1,7 @@
 // arith_check.hip - exhaustive on-device checks of the arithmetic shortcuts
-// the smoke advection's inverse-distance weights, the 16-bit storage
-// modes' paired roundings and the pressure sums take (fluid_device.hpp,
-// projection_device.hpp):
+// the smoke advection's inverse-distance weights and square roots, the
+// 16-bit storage modes' paired roundings and the pressure sums take
+// (fluid_device.hpp, projection_device.hpp):
 // each mode runs every one of the 2^32 binary32 bit patterns through the
 // shortcut and through the IEEE operations it replaces, on the GPU that
 // runs the solver, and counts the inputs where they differ.
@@ -27,6 +27,8 @@ __device__ __attribute__((noinline)) float inv_ieee(float s) {
 }
 
 __device__ __attribute__((noinline)) float div_ieee(float a, float b) { return a / b; }
+
+__device__ __attribute__((noinline)) float sqrt_ieee(float a) { return sqrtf(a); }
 
 // A numerator for denominator pattern b: b scaled by a factor in (0, 1]
 // from a hash of the pattern (an inverse-distance weight never exceeds the
@@ -69,6 +71,10 @@ __global__ void __launch_bounds__(kCheckThreads)
         const double e = __builtin_fma(-d, recip_nr(d), 1.0);
         bad = !(__builtin_fabs(e) <= 0x1p-50);
       }
+    } else if (mode == 7) {
+      // sqrt_staged against sqrtf on 0 and every finite input >= 2^-96
+      if (x == 0.0f || (x >= 0x1p-96f && x <= 0x1.fffffep127f))
+        bad = __float_as_uint(sqrt_staged(x)) != __float_as_uint(sqrt_ieee(x));
     } else if (mode == 5 || mode == 6) {
       // the pressure sums' quotient x / d_t as a double multiply by the
       // double nearest 1 / d_t, for the step sizes 0.05 and 1/60
@@ -113,7 +119,7 @@ using namespace osy;
 
 extern "C" int osy_arith_check(int32_t mode, uint64_t* mismatches, uint32_t* first_input) {
   if (!mismatches || !first_input) return set_error(OSY_E_INVALID, "null argument");
-  if (mode < 0 || mode > 6) return set_error(OSY_E_INVALID, "mode must be 0 to 6");
+  if (mode < 0 || mode > 7) return set_error(OSY_E_INVALID, "mode must be 0 to 7");
   void* buf = nullptr;
   OSY_HIP_CHECK(hipMalloc(&buf, 16));
   unsigned long long* count = (unsigned long long*)buf;

@@ -317,11 +317,30 @@ __device__ inline float inv_distance(float dx, float dy) {
   return inv_from_distance(sqrtf(dx * dx + dy * dy));
 }
 
-// inv_distance for a sample inside the staged region: its distances to the
+// sqrtf for t = 0 and every t >= 2^-96: the hardware square root (within
+// an ulp) corrected from the fused residuals t - s' * s of its two
+// neighbours s'.  Below 2^-96 the residuals are subnormal and lose the bits
+// the test needs, so smaller t are not in its domain.  Equal to sqrtf on
+// its domain (osy_arith_check mode 7).
+__device__ inline float sqrt_staged(float t) {
+  const float s = __builtin_amdgcn_sqrtf(t);
+  const float sm = __uint_as_float(__float_as_uint(s) - 1u);
+  const float sp = __uint_as_float(__float_as_uint(s) + 1u);
+  const float rm = __builtin_fmaf(-sm, s, t);
+  const float rp = __builtin_fmaf(-sp, s, t);
+  float r = rm <= 0.0f ? sm : s;
+  r = rp > 0.0f ? sp : r;
+  return r;
+}
+
+// inv_distance for a sample inside the staged region.  Its distances to the
 // four cell centres are a few cells at most, so the distance is finite and
-// the infinite case of inv_from_distance never arises.
+// the infinite case of inv_from_distance never arises; and the squared
+// distance is 0 or at least 2^-50 (a nonzero coordinate difference of two
+// positions of magnitude 1/4 or more is at least 2^-25), so never
+// subnormal.
 __device__ inline float inv_distance_staged(float dx, float dy) {
-  return (float)recip_nr((double)sqrtf(dx * dx + dy * dy) + 1e-6);
+  return (float)recip_nr((double)sqrt_staged(dx * dx + dy * dy) + 1e-6);
 }
 
 // inv / sum rounded to float (the weights of src/fluid.cu:700-712), given

@@ -146,7 +146,7 @@ def test_diagnostic_entry_points_reject_bad_arguments_before_device_work():
     lib = _abi.lib()
     n = ctypes.c_uint64()
     first = ctypes.c_uint32()
-    assert lib.osy_arith_check(7, ctypes.byref(n), ctypes.byref(first)) == _abi.OSY_E_INVALID
+    assert lib.osy_arith_check(8, ctypes.byref(n), ctypes.byref(first)) == _abi.OSY_E_INVALID
     assert b"mode" in lib.osy_last_error()
     assert lib.osy_arith_check(-1, ctypes.byref(n), ctypes.byref(first)) == _abi.OSY_E_INVALID
     assert lib.osy_arith_check(0, None, ctypes.byref(first)) == _abi.OSY_E_INVALID

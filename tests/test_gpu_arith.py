@@ -22,7 +22,8 @@ pytestmark = pytest.mark.gpu
                                         (3, "paired FP16 rounding vs scalar"),
                                         (4, "paired BF16 rounding vs scalar"),
                                         (5, "pressure quotient, d_t = 0.05"),
-                                        (6, "pressure quotient, d_t = 1/60")])
+                                        (6, "pressure quotient, d_t = 1/60"),
+                                        (7, "corrected square root vs sqrtf")])
 def test_arith_shortcut_exhaustive(mode, what):
     n = ctypes.c_uint64()
     first = ctypes.c_uint32()
